@@ -1,0 +1,132 @@
+/*
+ * nipet_hip.h -- C ABI of the MI355X (gfx950) implementation of NIPET's
+ * projection / reconstruction / list-mode histogramming path.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only. Every array argument documented "device"
+ *    is a device pointer on the library's current device; "host" arrays are
+ *    read (or written) before the call returns.
+ *  - Every entry point returns 0 on success and a negative NIPET_E* code on
+ *    failure; nipet_last_error() returns a message for the calling thread.
+ *    The library never exits the process (the reference's HandleError calls
+ *    exit(): niftypet/nipet/src/scanner_0.cu:13-18).
+ *  - `stream` is a hipStream_t passed as void* (NULL = the default stream).
+ *    Launches are asynchronous on that stream unless stated otherwise.
+ *  - Layouts are the reference's device layouts:
+ *      image   float32 [320 (y)][320 (x)][127 (z)], z fastest
+ *              (prj/src/prjf.cu:53-55);
+ *      sinogram float32 [nprj][nsn], sinogram index fastest, rows in the
+ *              order of the transaxial-bin list `subs` (prj/src/prjf.cu:90-94).
+ */
+#ifndef NIPET_HIP_H
+#define NIPET_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NIPET_OK 0
+#define NIPET_EINVAL (-1)   /* bad argument / shape */
+#define NIPET_EHIP (-2)     /* HIP runtime error */
+#define NIPET_ENOMEM (-3)   /* device allocation failed */
+#define NIPET_ESTATE (-4)   /* object in wrong state */
+
+typedef struct NipetGeom NipetGeom;
+
+/* Scanner geometry handed over once per session. All pointers are host.
+ * Replaces the per-call LUT upload + transaxial tracing of the reference
+ * (prj/src/prj_module.cu:380-398, prj/src/tprj.cu:13-200). */
+typedef struct {
+  const float *crs;      /* [ncrs][4] crystal face end points x0,y0,x1,y1 [cm] (txLUT['crs']) */
+  int ncrs;              /* 504 */
+  const int16_t *s2c;    /* [naw][2] crystal pair of each transaxial bin (txLUT['s2c']) */
+  int naw;               /* 68516 */
+  const float *li2rng;   /* [nli][2] axial z start [cm] of ring r0, r1 (axLUT['li2rng']) */
+  const int16_t *li2sn;  /* [nli][2] output sinogram of LOR r0->r1 and of its mirror
+                            (axLUT['li2sn'] for span-11, ['li2sn1'] for span-1) */
+  const int8_t *li2nos;  /* [nli] span-1 LORs per output sinogram (axLUT['li2nos']) */
+  int nli;               /* 2074; the first ndirect entries are direct planes */
+  int nsn;               /* 837 (span-11) or 4084 (span-1) */
+  float ring_pitch;      /* 0.40625 cm (SZ_RING) */
+  float vox_xy;          /* 0.208626 cm (SZ_VOXY) */
+  float vox_z;           /* 0.203125 cm (SZ_VOXZ) */
+  float tfov2;           /* 890 cm^2 (TFOV2) */
+} NipetGeomDesc;
+
+/* Build the geometry on the current device: the FP64 transaxial intersection
+ * lists of all naw bins (kept device-resident in CSR form) and the axial tables.
+ * Synchronous. Reference: petprj setup of frwd_prj/back_prj/osem_rec
+ * (prj/src/prj_module.cu:232-425, 430-608, 613-839) + sddn_tx
+ * (prj/src/tprj.cu:13-172). */
+int nipet_geom_create(const NipetGeomDesc *desc, NipetGeom **out);
+void nipet_geom_destroy(NipetGeom *g);
+
+/* Sizes of the built geometry (host outputs, any may be NULL). */
+int nipet_geom_info(const NipetGeom *g, int *naw, int64_t *nseg_total, int *nli, int *nsn);
+
+/* Copy the transaxial intersection lists to host arrays: offsets[naw+1],
+ * col[nseg_total] (voxel column u + 320 v), t[nseg_total] (segment start, cm),
+ * len[nseg_total] (segment length, cm). The equivalent of the reference's
+ * tv/tt tables (petprj.tprj, prj/src/prj_module.cu:70-224). Synchronous. */
+int nipet_tx_lists(const NipetGeom *g, int64_t *offsets, int32_t *col, float *t, float *len);
+
+/* Forward projection (petprj.fprj, prj/src/prj_module.cu:232-425;
+ * gpu_fprj prj/src/prjf.cu:209-320).
+ *   sino [nprj][nsn] float (device, fully overwritten), img (device),
+ *   subs [nprj] int32 transaxial bins (device), or NULL for all naw bins in
+ *   order (the reference's isub = [-1]).
+ *   att = 0: emission (bins sum their span-1 LORs);
+ *   att = 1: attenuation factors exp(-line integral), bins average their LORs. */
+int nipet_fprj(const NipetGeom *g, float *sino, const float *img, const int32_t *subs, int nprj,
+               int att, void *stream);
+
+/* Back projection (petprj.bprj, prj/src/prj_module.cu:430-608;
+ * gpu_bprj prj/src/prjb.cu:190-308): img (device) is overwritten with the
+ * adjoint of the emission projector applied to sino [nprj][nsn] (device). */
+int nipet_bprj(const NipetGeom *g, float *img, const float *sino, const int32_t *subs, int nprj,
+               void *stream);
+
+/* Gap removal: Siemens sinogram in[nsn][nbinang] (nbinang = 252*344; float32
+ * when in_dtype = 0, uint16 when 1) -> device sinogram out[naw][nsn] float32,
+ * out[aw][s] = in[s][aw2ali[aw]]. All device pointers. Replaces
+ * mmr_auxe.rgaps / remove_gaps (src/scanner_0.cu:94-155, mmraux.py:1146-1160). */
+int nipet_remgaps(float *out, const void *in, int in_dtype, const int32_t *aw2ali, int naw,
+                  int nsn, int nbinang, void *stream);
+
+/* Gap insertion: device sinogram in[naw][nsn] -> Siemens out[nsn][nbinang]
+ * (gaps zero), out[s][aw2ali[aw]] = in[aw][s]. Replaces mmr_auxe.pgaps /
+ * put_gaps (src/scanner_0.cu:158-241) and the transpose of mmraux.py:1141. */
+int nipet_putgaps(float *out, const float *in, const int32_t *aw2ali, int naw, int nsn,
+                  int nbinang, void *stream);
+
+/* OSEM corrections, computed once per reconstruction (prj/src/recon.cu:381-398):
+ *   rho[i] = (r[i] + s[i]) / (a[i] n[i])  (0 where |a n| < 1e-6),  an[i] = a[i] n[i].
+ * r or s may be NULL (zero). All device arrays of n elements. */
+int nipet_osem_corr(float *rho, float *an, const float *a, const float *n, const float *r,
+                    const float *s, int64_t nelem, void *stream);
+
+/* One OSEM subset update (the body of the subset loop of petprj.osem,
+ * prj/src/recon.cu:459-497), all device pointers:
+ *   e = FP_sub(img) + rho[subs]; e = y[subs] / e (0 where |e| < 1e-6);
+ *   b = BP_sub(e); img = mask ? img * (b / sens) : 0 (b/sens = 0 where |sens| < 1e-6).
+ * prompts: uint16 [naw][nsn]; rho: float [naw][nsn]; sens: float image of this
+ * subset; mask: uint8 image; work_sino: float [nprj][nsn]; work_img: float image. */
+int nipet_osem_subset(const NipetGeom *g, float *img, const uint16_t *prompts, const float *rho,
+                      const float *sens, const uint8_t *mask, const int32_t *subs, int nprj,
+                      float *work_sino, float *work_img, void *stream);
+
+/* Number of list-mode words per chunk that nipet_lm_hist accepts in one call. */
+#define NIPET_LM_MAXWORDS (1u << 30)
+
+const char *nipet_last_error(void);
+
+/* Library version string. */
+const char *nipet_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NIPET_HIP_H */

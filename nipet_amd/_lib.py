@@ -1,0 +1,87 @@
+"""ctypes binding of ``libnipet_hip.so`` (C ABI declared in ``include/nipet_hip.h``).
+
+The library is loaded from this package directory only. There is no CPU
+fallback: if the library is missing or the GPU is unavailable every entry
+point raises.
+"""
+import ctypes as ct
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, 'libnipet_hip.so')
+_lock = threading.Lock()
+_lib = None
+
+# every symbol include/nipet_hip.h declares
+EXPORTS = (
+    'nipet_geom_create', 'nipet_geom_destroy', 'nipet_geom_info', 'nipet_tx_lists', 'nipet_fprj',
+    'nipet_bprj', 'nipet_osem_corr', 'nipet_osem_subset', 'nipet_lm_hist', 'nipet_lm_tags',
+    'nipet_philox4x32', 'nipet_last_error', 'nipet_version')
+
+
+class NipetError(RuntimeError):
+    pass
+
+
+class NipetGeomDesc(ct.Structure):
+    _fields_ = [
+        ('crs', ct.c_void_p), ('ncrs', ct.c_int), ('s2c', ct.c_void_p), ('naw', ct.c_int),
+        ('li2rng', ct.c_void_p), ('li2sn', ct.c_void_p), ('li2nos', ct.c_void_p),
+        ('nli', ct.c_int), ('nsn', ct.c_int), ('ring_pitch', ct.c_float),
+        ('vox_xy', ct.c_float), ('vox_z', ct.c_float), ('tfov2', ct.c_float)]
+
+
+def lib():
+    """Load the shared library (raises NipetError when it is not built)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIBPATH):
+            raise NipetError(f'{LIBPATH} is not built: run __graft_entry__.build() or '
+                             '`python -m nipet_amd._build`')
+        L = ct.CDLL(LIBPATH)
+        P, I, I64 = ct.c_void_p, ct.c_int, ct.c_int64
+        sig = {
+            'nipet_geom_create': ([ct.POINTER(NipetGeomDesc), ct.POINTER(P)], I),
+            'nipet_geom_destroy': ([P], None),
+            'nipet_geom_info': ([P, P, P, P, P], I),
+            'nipet_tx_lists': ([P, P, P, P, P], I),
+            'nipet_fprj': ([P, P, P, P, I, I, P], I),
+            'nipet_bprj': ([P, P, P, P, I, P], I),
+            'nipet_osem_corr': ([P, P, P, P, P, P, I64, P], I),
+            'nipet_remgaps': ([P, P, I, P, I, I, I, P], I),
+            'nipet_putgaps': ([P, P, P, I, I, I, P], I),
+            'nipet_osem_subset': ([P, P, P, P, P, P, P, I, P, P, P], I),
+            'nipet_last_error': ([], ct.c_char_p),
+            'nipet_version': ([], ct.c_char_p),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+        return L
+
+
+def check(rc, what=''):
+    if rc != 0:
+        msg = lib().nipet_last_error().decode(errors='replace')
+        raise NipetError(f'{what}: {msg} (code {rc})')
+
+
+def ptr(t):
+    """Device (or host) address of a torch tensor / numpy array, or None."""
+    if t is None:
+        return None
+    if isinstance(t, np.ndarray):
+        return t.ctypes.data
+    return t.data_ptr()
+
+
+def stream_handle(device=None):
+    import torch
+    return torch.cuda.current_stream(device).cuda_stream

@@ -1,0 +1,54 @@
+// Shared definitions of the gfx950 NIPET library (device image / sinogram
+// geometry, the geometry object, error reporting).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nipet_hip.h"
+
+namespace nipet {
+
+constexpr int NX = 320;   // device image x (u)
+constexpr int NY = 320;   // device image y (v)
+constexpr int NZ = 127;   // device image z (w), fastest
+constexpr int64_t NVOX = (int64_t)NX * NY * NZ;
+constexpr int MAXNSN = 4096;  // largest sinogram count handled (span-1: 4084)
+constexpr float EPS = 1e-6f;  // OSEM division guard (prj/src/recon.cu:12)
+
+// One transaxial segment: start t (cm from the first crystal, double so that
+// z-boundary positions along the ray keep sub-nanometre precision), length
+// (cm) and the z-column base offset 127*(u + 320 v) of its voxel column.
+struct __align__(16) SegRec {
+  double s;
+  float len;
+  int32_t colz;
+};
+
+// Geometry resident on one device. Transaxial intersection lists in CSR form:
+// segments of ray aw are seg[off[aw] .. off[aw+1]).
+struct Geom {
+  int device = 0;
+  int naw = 0, nli = 0, nsn = 0, ncrs = 0, ndirect = 0;
+  int64_t nseg = 0;
+  float vox_z = 0.203125f;
+  int32_t *off = nullptr;    // [naw+1]
+  SegRec *seg = nullptr;     // [nseg]
+  double *atn = nullptr;     // [naw] crystal-to-crystal transaxial distance
+  float *lmax = nullptr;     // [naw] longest segment of the ray
+  float2 *li_z = nullptr;    // [nli] (zeta_a at t=0, zeta_b - zeta_a) in z-voxel units
+  short2 *li_sn = nullptr;   // [nli] output sinogram of LOR a and of its mirror b
+  float *li_nosi = nullptr;  // [nli] 1 / (span-1 LORs per output sinogram)
+};
+
+void set_error(const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what);
+
+}  // namespace nipet
+
+struct NipetGeom : nipet::Geom {};
+
+#define NIPET_HIP(call)                                              \
+  do {                                                               \
+    hipError_t e_ = (call);                                          \
+    if (e_ != hipSuccess) return nipet::hip_fail(e_, #call);         \
+  } while (0)

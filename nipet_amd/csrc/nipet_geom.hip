@@ -1,0 +1,364 @@
+// Geometry: FP64 transaxial intersection lists of the mMR, built once per
+// session on the device, plus the axial (Michelogram) tables.
+//
+// Semantics of a transaxial list (reference: sddn_tx prj/src/tprj.cu:13-172
+// and the segment walk of prj/src/prjf.cu:59-86):
+//  - the ray joins the centres of the faces of crystals s2c[aw][0] and
+//    s2c[aw][1]; t is the transaxial distance from the first crystal;
+//  - the ray is clipped to the FOV circle x^2+y^2 = tfov2: t in (t1, t2);
+//  - every crossing of a grid line x = i*vox or y = j*vox strictly inside
+//    (t1, t2) is a boundary; the pieces before the first and after the last
+//    crossing are not part of the list; a list entry is the voxel between two
+//    consecutive crossings;
+//  - crossings are merged in increasing t (x-line first on an exact tie); the
+//    voxel index is stepped by +-1 at each crossing starting from the voxel
+//    holding the midpoint of [t1, first crossing];
+//  - u = 160 + floor(x/vox), v = 160 - ceil(y/vox) (prj/src/tprj.cu:62,82).
+// Everything is computed in double without contraction, so the lists are
+// bit-identical to a straightforward double-precision CPU tracer.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "nipet_common.h"
+
+namespace nipet {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  set_error("HIP error %d (%s) in %s", (int)e, hipGetErrorString(e), what);
+  return (e == hipErrorOutOfMemory) ? NIPET_ENOMEM : NIPET_EHIP;
+}
+
+#pragma clang fp contract(off)
+
+struct RayD {
+  double px, py, ax, ay, atn, t1, t2;
+};
+
+__device__ static void ray_setup(const float *crs, int c1, int c2, double tfov2, RayD &r) {
+  double x1 = 0.5 * ((double)crs[4 * c1 + 0] + (double)crs[4 * c1 + 2]);
+  double y1 = 0.5 * ((double)crs[4 * c1 + 1] + (double)crs[4 * c1 + 3]);
+  double x2 = 0.5 * ((double)crs[4 * c2 + 0] + (double)crs[4 * c2 + 2]);
+  double y2 = 0.5 * ((double)crs[4 * c2 + 1] + (double)crs[4 * c2 + 3]);
+  double dx = x2 - x1, dy = y2 - y1;
+  double atn = sqrt(dx * dx + dy * dy);
+  r.px = x1;
+  r.py = y1;
+  r.ax = dx / atn;
+  r.ay = dy / atn;
+  r.atn = atn;
+  double b = x1 * r.ax + y1 * r.ay;
+  double c = x1 * x1 + y1 * y1 - tfov2;
+  double disc = b * b - c;
+  double sq = disc > 0 ? sqrt(disc) : 0.0;
+  r.t1 = -b - sq;
+  r.t2 = -b + sq;
+}
+
+// Crossings of the lines coord = i*vox in increasing t.
+struct LineWalk {
+  double p, a, vox, t1, t2, t;
+  long i, iend;
+  int dir;
+  bool live;
+
+  __device__ void init(double p_, double a_, double vox_, double t1_, double t2_) {
+    p = p_;
+    a = a_;
+    vox = vox_;
+    t1 = t1_;
+    t2 = t2_;
+    live = (a != 0.0);
+    if (!live) return;
+    double c1 = p + a * t1, c2 = p + a * t2;
+    double lo = c1 < c2 ? c1 : c2, hi = c1 < c2 ? c2 : c1;
+    long i0 = (long)floor(lo / vox) - 1, i1 = (long)ceil(hi / vox) + 1;
+    if (a > 0) {
+      i = i0;
+      iend = i1;
+      dir = 1;
+    } else {
+      i = i1;
+      iend = i0;
+      dir = -1;
+    }
+    // skip to the first crossing with t > t1
+    for (;;) {
+      t = ((double)i * vox - p) / a;
+      if (t > t1) break;
+      if (i == iend) {
+        live = false;
+        return;
+      }
+      i += dir;
+    }
+    live = (t < t2);
+  }
+  __device__ void advance() {
+    if (i == iend) {
+      live = false;
+      return;
+    }
+    i += dir;
+    t = ((double)i * vox - p) / a;
+    live = (t > t1 && t < t2);
+  }
+};
+
+// One thread per transaxial bin. Pass 1 (col == nullptr): counts. Pass 2:
+// writes the segment records at off[aw].
+__global__ void k_tx_lists(const float *crs, const short2 *s2c, int naw, double vox,
+                           double tfov2, int nz, const int32_t *off, int32_t *count,
+                           SegRec *seg, double *atn_out, float *lmax_out, int *err) {
+  int aw = blockIdx.x * blockDim.x + threadIdx.x;
+  if (aw >= naw) return;
+  short2 cc = s2c[aw];
+  RayD r;
+  ray_setup(crs, cc.x, cc.y, tfov2, r);
+  LineWalk wx, wy;
+  wx.init(r.px, r.ax, vox, r.t1, r.t2);
+  wy.init(r.py, r.ay, vox, r.t1, r.t2);
+  int n = 0;
+  if (wx.live || wy.live) {
+    bool xfirst = wx.live && (!wy.live || wx.t <= wy.t);
+    double tf = xfirst ? wx.t : wy.t;
+    double tm = 0.5 * (r.t1 + tf);
+    long u = NX / 2 + (long)floor((r.px + r.ax * tm) / vox);
+    long v = NY / 2 - (long)ceil((r.py + r.ay * tm) / vox);
+    int su = r.ax > 0 ? 1 : -1;
+    int sv = r.ay > 0 ? 1 : -1;
+    double tprev = 0;
+    bool first = true;
+    int32_t base = seg ? off[aw] : 0;
+    int32_t cap = seg ? off[aw + 1] - base : 0;
+    float lmax = 0.f;
+    while (wx.live || wy.live) {
+      bool takex = wx.live && (!wy.live || wx.t <= wy.t);
+      double t = takex ? wx.t : wy.t;
+      if (!first) {
+        if (seg && n < cap) {
+          SegRec rec;
+          rec.s = tprev;
+          rec.len = (float)(t - tprev);
+          rec.colz = (int32_t)(nz * (u + NX * v));
+          seg[base + n] = rec;
+          lmax = fmaxf(lmax, rec.len);
+        }
+        n++;
+      }
+      if (takex) {
+        u += su;
+        wx.advance();
+      } else {
+        v -= sv;
+        wy.advance();
+      }
+      if (u < 0 || u >= NX || v < 0 || v >= NY) {
+        atomicOr(err, 1);
+        return;
+      }
+      tprev = t;
+      first = false;
+    }
+    if (seg) lmax_out[aw] = lmax;
+  }
+  if (!seg) {
+    count[aw] = n;
+    atn_out[aw] = r.atn;
+  }
+}
+
+}  // namespace nipet
+
+using namespace nipet;
+
+extern "C" {
+
+const char *nipet_last_error(void) { return g_err; }
+
+const char *nipet_version(void) { return "nipet_amd 0.1 (gfx950)"; }
+
+static void geom_free(Geom *g) {
+  hipFree(g->off);
+  hipFree(g->seg);
+  hipFree(g->atn);
+  hipFree(g->lmax);
+  hipFree(g->li_z);
+  hipFree(g->li_sn);
+  hipFree(g->li_nosi);
+}
+
+static int geom_build(const NipetGeomDesc *d, NipetGeom *g) {
+  NIPET_HIP(hipGetDevice(&g->device));
+  g->naw = d->naw;
+  g->nli = d->nli;
+  g->nsn = d->nsn;
+  g->ncrs = d->ncrs;
+  g->vox_z = d->vox_z;
+
+  // axial tables: zeta (z in voxel units, 0 at the bottom face of plane 0)
+  // of the ring centres of LOR a (r0 -> r1); the mirror b runs r1 -> r0.
+  std::vector<float2> liz(d->nli);
+  std::vector<short2> lisn(d->nli);
+  std::vector<float> linos(d->nli);
+  int ndirect = 0;
+  for (int li = 0; li < d->nli; li++) {
+    double za = (double)d->li2rng[2 * li] + 0.5 * d->ring_pitch;
+    double zb = (double)d->li2rng[2 * li + 1] + 0.5 * d->ring_pitch;
+    double zeta_a = za / d->vox_z + 0.5 * NZ;
+    double zeta_b = zb / d->vox_z + 0.5 * NZ;
+    if (zeta_a < 0 || zeta_a >= NZ || zeta_b < 0 || zeta_b >= NZ || zeta_b < zeta_a) {
+      set_error("li2rng[%d] outside the image or with r1 < r0", li);
+      return NIPET_EINVAL;
+    }
+    liz[li] = make_float2((float)zeta_a, (float)(zeta_b - zeta_a));
+    int sa = d->li2sn[2 * li], sb = d->li2sn[2 * li + 1];
+    if (sa < 0 || sa >= d->nsn || sb < 0 || sb >= d->nsn || d->li2nos[li] <= 0) {
+      set_error("li2sn/li2nos[%d] out of range", li);
+      return NIPET_EINVAL;
+    }
+    lisn[li] = make_short2((short)sa, (short)sb);
+    linos[li] = 1.0f / (float)d->li2nos[li];
+    if (zeta_b == zeta_a) {
+      if (li != ndirect) {
+        set_error("direct planes must come first in the li tables (li=%d)", li);
+        return NIPET_EINVAL;
+      }
+      ndirect++;
+    }
+  }
+  g->ndirect = ndirect;
+  NIPET_HIP(hipMalloc(&g->li_z, d->nli * sizeof(float2)));
+  NIPET_HIP(hipMalloc(&g->li_sn, d->nli * sizeof(short2)));
+  NIPET_HIP(hipMalloc(&g->li_nosi, d->nli * sizeof(float)));
+  NIPET_HIP(hipMemcpy(g->li_z, liz.data(), d->nli * sizeof(float2), hipMemcpyHostToDevice));
+  NIPET_HIP(hipMemcpy(g->li_sn, lisn.data(), d->nli * sizeof(short2), hipMemcpyHostToDevice));
+  NIPET_HIP(hipMemcpy(g->li_nosi, linos.data(), d->nli * sizeof(float), hipMemcpyHostToDevice));
+
+  // transaxial lists
+  float *d_crs = nullptr;
+  short2 *d_s2c = nullptr;
+  int32_t *d_cnt = nullptr;
+  int *d_err = nullptr;
+  NIPET_HIP(hipMalloc(&d_crs, d->ncrs * 4 * sizeof(float)));
+  NIPET_HIP(hipMalloc(&d_s2c, d->naw * sizeof(short2)));
+  NIPET_HIP(hipMalloc(&d_cnt, d->naw * sizeof(int32_t)));
+  NIPET_HIP(hipMalloc(&d_err, sizeof(int)));
+  NIPET_HIP(hipMalloc(&g->atn, d->naw * sizeof(double)));
+  NIPET_HIP(hipMalloc(&g->lmax, d->naw * sizeof(float)));
+  NIPET_HIP(hipMalloc(&g->off, (d->naw + 1) * sizeof(int32_t)));
+  NIPET_HIP(hipMemcpy(d_crs, d->crs, d->ncrs * 4 * sizeof(float), hipMemcpyHostToDevice));
+  NIPET_HIP(hipMemcpy(d_s2c, d->s2c, d->naw * sizeof(short2), hipMemcpyHostToDevice));
+  NIPET_HIP(hipMemset(d_err, 0, sizeof(int)));
+  NIPET_HIP(hipMemset(g->lmax, 0, d->naw * sizeof(float)));
+  int nb = (d->naw + 255) / 256;
+  k_tx_lists<<<nb, 256>>>(d_crs, d_s2c, d->naw, (double)d->vox_xy, (double)d->tfov2, NZ,
+                          nullptr, d_cnt, nullptr, g->atn, g->lmax, d_err);
+  NIPET_HIP(hipGetLastError());
+  std::vector<int32_t> cnt(d->naw);
+  int herr = 0;
+  NIPET_HIP(hipMemcpy(cnt.data(), d_cnt, d->naw * sizeof(int32_t), hipMemcpyDeviceToHost));
+  NIPET_HIP(hipMemcpy(&herr, d_err, sizeof(int), hipMemcpyDeviceToHost));
+  if (herr) {
+    set_error("transaxial ray left the image grid (inconsistent geometry)");
+    return NIPET_EINVAL;
+  }
+  std::vector<int32_t> off(d->naw + 1);
+  int64_t tot = 0;
+  for (int i = 0; i < d->naw; i++) {
+    off[i] = (int32_t)tot;
+    tot += cnt[i];
+  }
+  off[d->naw] = (int32_t)tot;
+  if (tot >= INT32_MAX) {
+    set_error("too many transaxial segments");
+    return NIPET_EINVAL;
+  }
+  g->nseg = tot;
+  NIPET_HIP(hipMemcpy(g->off, off.data(), (d->naw + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+  NIPET_HIP(hipMalloc(&g->seg, (tot + 1) * sizeof(SegRec)));
+  k_tx_lists<<<nb, 256>>>(d_crs, d_s2c, d->naw, (double)d->vox_xy, (double)d->tfov2, NZ, g->off,
+                          nullptr, g->seg, nullptr, g->lmax, d_err);
+  NIPET_HIP(hipGetLastError());
+  NIPET_HIP(hipDeviceSynchronize());
+  hipFree(d_crs);
+  hipFree(d_s2c);
+  hipFree(d_cnt);
+  hipFree(d_err);
+  return NIPET_OK;
+}
+
+int nipet_geom_create(const NipetGeomDesc *d, NipetGeom **out) {
+  if (!d || !out || !d->crs || !d->s2c || !d->li2rng || !d->li2sn || !d->li2nos) {
+    set_error("nipet_geom_create: NULL argument");
+    return NIPET_EINVAL;
+  }
+  if (d->naw <= 0 || d->nli <= 0 || d->nsn <= 0 || d->nsn > MAXNSN || d->ncrs <= 0 ||
+      d->vox_xy <= 0 || d->vox_z <= 0 || d->tfov2 <= 0) {
+    set_error("nipet_geom_create: bad sizes (naw=%d nli=%d nsn=%d)", d->naw, d->nli, d->nsn);
+    return NIPET_EINVAL;
+  }
+  for (int i = 0; i < 2 * d->naw; i++)
+    if (d->s2c[i] < 0 || d->s2c[i] >= d->ncrs) {
+      set_error("nipet_geom_create: s2c entry %d out of range", i);
+      return NIPET_EINVAL;
+    }
+  NipetGeom *g = new NipetGeom();
+  int rc = geom_build(d, g);
+  if (rc != NIPET_OK) {
+    geom_free(g);
+    delete g;
+    return rc;
+  }
+  *out = g;
+  return NIPET_OK;
+}
+
+void nipet_geom_destroy(NipetGeom *g) {
+  if (!g) return;
+  geom_free(g);
+  delete g;
+}
+
+int nipet_geom_info(const NipetGeom *g, int *naw, int64_t *nseg_total, int *nli, int *nsn) {
+  if (!g) {
+    set_error("nipet_geom_info: NULL geometry");
+    return NIPET_EINVAL;
+  }
+  if (naw) *naw = g->naw;
+  if (nseg_total) *nseg_total = g->nseg;
+  if (nli) *nli = g->nli;
+  if (nsn) *nsn = g->nsn;
+  return NIPET_OK;
+}
+
+int nipet_tx_lists(const NipetGeom *g, int64_t *offsets, int32_t *col, float *t, float *len) {
+  if (!g || !offsets || !col || !t || !len) {
+    set_error("nipet_tx_lists: NULL argument");
+    return NIPET_EINVAL;
+  }
+  std::vector<int32_t> off(g->naw + 1);
+  std::vector<SegRec> sg(g->nseg);
+  NIPET_HIP(hipMemcpy(off.data(), g->off, (g->naw + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+  NIPET_HIP(hipMemcpy(sg.data(), g->seg, g->nseg * sizeof(SegRec), hipMemcpyDeviceToHost));
+  for (int i = 0; i <= g->naw; i++) offsets[i] = off[i];
+  for (int64_t k = 0; k < g->nseg; k++) {
+    col[k] = sg[k].colz / NZ;
+    t[k] = (float)sg[k].s;
+    len[k] = sg[k].len;
+  }
+  return NIPET_OK;
+}
+
+}  // extern "C"

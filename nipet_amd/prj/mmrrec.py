@@ -1,0 +1,151 @@
+"""OSEM reconstruction driver (``niftypet/nipet/prj/mmrrec.py``).
+
+* ``get_subsets14`` -- ``prj/mmrrec.py:43-93``: 14 balanced angular subsets of
+  18 angles each (14 angles taken one per 18-angle block in a rotating pattern
+  plus 4 of the remaining angles), returned as transaxial-bin (aw) indices in
+  (radial bin, angle) order.
+* ``psf_config``    -- ``prj/mmrrec.py:96-155``.
+* ``osemone``       -- ``prj/mmrrec.py:158-470`` (the corrections are given as
+  arrays: this path has no scanner-file readers, randoms or scatter models).
+"""
+import logging
+import time
+from collections.abc import Iterable
+from numbers import Real
+
+import numpy as np
+
+log = logging.getLogger(__name__)
+
+__all__ = ['get_subsets14', 'psf_config', 'fwhm2sig', 'osemone', 'osem_prepare', 'osem_run']
+
+
+def fwhm2sig(fwhm, voxsize=1.):
+    return (fwhm/voxsize) / (2 * (2 * np.log(2))**.5)
+
+
+def get_subsets14(n, params):
+    """The n-th of 14 transaxial subsets: (aw indices, angle table S (14,18))."""
+    Cnt, txLUT = params['Cnt'], params['txLUT']
+    N = 14
+    NSA = Cnt['NSANGLES']
+    P = NSA // N                    # 18 angles per subset / per block
+    fs = N / float(P - N)           # spread of the 4 remaining angles per block
+    blocks = np.array([np.arange(i, NSA, P) for i in range(N)])   # (14 offsets, 14 blocks)
+    S = np.zeros((N, P), dtype=np.int16)
+    for s in range(N):
+        angles = [blocks[(s + b) % N, b] for b in range(N)]
+        extra = np.int16(np.floor(np.arange(s, 2 * N, fs)[:P - N] % N))
+        for i in range(P - N):
+            angles.append(blocks[-1, extra[i]] + i + 1)
+        S[s] = angles
+    live = txLUT['msino'] > 0
+    awmap = -np.ones(live.shape, dtype=np.int32)
+    awmap[live] = np.arange(Cnt['Naw'], dtype=np.int32)
+    iprj = awmap[:, S[n]]
+    iprj = iprj[iprj >= 0]
+    return iprj, S
+
+
+def psf_config(psf, Cnt):
+    """Separable PSF kernel (3, 17) ordered (z, y, x); ``[0,0] < 0`` switches
+    resolution modelling off (``prj/mmrrec.py:96-155``)."""
+    R = Cnt['RSZ_PSF_KRNL']
+
+    def _config(fwhm3, check_len=True):
+        if check_len and (len(fwhm3) != 3 or any(f < 0 for f in fwhm3)):
+            raise ValueError('Incorrect separable kernel FWHM definition')
+        kernel = np.empty((3, 2*R + 1), dtype=np.float32)
+        for i, f in enumerate(fwhm3):
+            vox = Cnt['SZ_VOXZ'] * 10 if i == 2 else Cnt['SZ_VOXY'] * 10
+            sig = fwhm2sig(f, voxsize=vox)
+            x = np.arange(-R, R + 1)
+            kernel[i] = np.exp(-0.5 * (x**2 / sig**2))
+            kernel[i] /= np.sum(kernel[i])
+        out = np.empty((3, 2*R + 1), dtype=np.float32)
+        out[0], out[1], out[2] = kernel[2], kernel[0], kernel[1]
+        return out
+
+    if psf is None:
+        k = _config([], False)
+        k[0, 0] = -1
+        return k
+    if isinstance(psf, str):
+        raise ValueError("measured PSF kernels need the scanner's calibration (not available)")
+    if isinstance(psf, Real):
+        return _config([psf] * 3)
+    if isinstance(psf, Iterable):
+        psf = np.asanyarray(psf)
+        if psf.shape == (3, 2*R + 1):
+            k = _config([], False)
+            k[0], k[1], k[2] = psf[2], psf[0], psf[1]
+            return k
+        if len(psf) == 3:
+            return _config(psf)
+        raise ValueError(f"invalid PSF dimensions ({psf.shape})")
+    raise ValueError(f"unrecognised PSF definition ({psf})")
+
+
+def osem_prepare(*args, **kwargs):
+    from .osem import osem_prepare as f
+    return f(*args, **kwargs)
+
+
+def osem_run(*args, **kwargs):
+    from .osem import osem_run as f
+    return f(*args, **kwargs)
+
+
+def osemone(datain, mumaps, hst, scanner_params, recmod=3, itr=4, fwhm=0., psf=None,
+            mask_radius=29., decay_ref_time=None, attnsino=None, sctsino=None, randsino=None,
+            normcomp=None, emmskS=False, frmno='', fcomment='', outpath=None, fout=None,
+            store_img=False, store_itr=None, ret_sinos=False, normsino=None):
+    """OSEM reconstruction (``prj/mmrrec.py:158-470``) on the GPU.
+
+    ``hst['psino']`` holds the prompts (Siemens (837,252,344) or device
+    (68516,837)). Corrections are arrays in either shape: ``attnsino``
+    (attenuation factors; computed from ``mumaps`` by the attenuation forward
+    projector when None and recmod>0), ``normsino`` (normalisation factors,
+    ones when None), ``randsino`` and ``sctsino`` (additive; zero when None).
+    Returns a namedtuple ``(im, fpet, imsmo, fsmo, affine)`` with the E7 image.
+    """
+    from collections import namedtuple
+
+    from ..img import mmrimg
+    from . import osem as _osem
+    from . import mmrprj
+    Cnt, txLUT = scanner_params['Cnt'], scanner_params['txLUT']
+
+    def dev(s):
+        if s is None:
+            return None
+        s = np.asarray(s)
+        if s.ndim == 3:
+            from ..mmraux import remgaps
+            return remgaps(s, txLUT, Cnt)
+        return s
+
+    psng = dev(hst['psino'])
+    if recmod == 0:
+        asng = np.ones(psng.shape, dtype=np.float32)
+    elif attnsino is not None:
+        asng = dev(attnsino).astype(np.float32)
+    else:
+        muh, muo = mumaps
+        mus = mmrimg.convert2dev(np.asarray(muo) + np.asarray(muh), Cnt)
+        asng = mmrprj.frwd_prj(mus, scanner_params, dev_out=True, attenuation=True)
+        asng = np.asarray(asng.cpu() if hasattr(asng, 'cpu') else asng)
+    nsng = np.ones(psng.shape, np.float32) if normsino is None else dev(normsino)
+    rsng = None if randsino is None else dev(randsino)
+    ssng = None if (sctsino is None or recmod < 2) else dev(sctsino)
+    psfkernel = psf_config(psf, Cnt)
+    if psfkernel[0, 0] >= 0:
+        raise NotImplementedError('PSF resolution modelling is not part of this path yet')
+    stime = time.time()
+    state = _osem.osem_prepare(scanner_params, psng, asng, nsng, rsng, ssng,
+                               mask_radius=mask_radius)
+    img = _osem.osem_run(state, itr)
+    log.info('recon time: %.3g', time.time() - stime)
+    im = mmrimg.convert2e7(img, Cnt)
+    RecOut = namedtuple('RecOut', 'im, fpet, imsmo, fsmo, affine')
+    return RecOut(im, None, None, None, None)

@@ -1,0 +1,125 @@
+"""ctypes wrapper of the FP64 CPU oracle (``oracle/nipet_oracle.c``).
+
+Test infrastructure only: imported by ``tests/``, ``__graft_entry__.smoke()``
+and ``bench.py``'s CPU-baseline leg, never by the ``nipet_amd`` package.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+NZ = 127
+NVOX = 320 * 320 * 127
+
+
+def build():
+    import subprocess
+    subprocess.run(['make', '-s', '-C', _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, 'liboracle.so')
+        if not os.path.exists(path):
+            build()
+        L = ct.CDLL(path)
+        P = ct.c_void_p
+        L.orc_tx_lists.argtypes = [P, P, P, ct.c_int, P, P, P, P, P, P]
+        L.orc_fprj.argtypes = [P, P, P, ct.c_int, P, P, P, P, P, ct.c_int, ct.c_int, ct.c_int, P]
+        L.orc_bprj.argtypes = [P, P, P, ct.c_int, P, P, P, P, ct.c_int, ct.c_int, ct.c_int]
+        for f in (L.orc_tx_lists, L.orc_fprj, L.orc_bprj):
+            f.restype = ct.c_int
+        L.orc_max_threads.restype = ct.c_int
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(ct.c_void_p)
+
+
+def _geom(params):
+    tx, ax = params['txLUT'], params['axLUT']
+    crs = np.ascontiguousarray(tx['crs'], dtype=np.float32)
+    s2c = np.ascontiguousarray(tx['s2c'], dtype=np.int16)
+    li2rng = np.ascontiguousarray(ax['li2rng'], dtype=np.float32)
+    span = params['Cnt']['SPN']
+    li2sn = np.ascontiguousarray(ax['li2sn'] if span == 11 else ax['li2sn1'], dtype=np.int16)
+    li2nos = np.ascontiguousarray(ax['li2nos'] if span == 11 else np.ones(len(li2rng)),
+                                  dtype=np.int8)
+    nsn = params['Cnt']['NSN11'] if span == 11 else params['Cnt']['NSN1']
+    return crs, s2c, li2rng, li2sn, li2nos, nsn
+
+
+def tx_lists(params, rays):
+    """Segment lists of the given transaxial bins.
+    Returns (offsets int64, col int32, t double, len double, atn double, t12 double)."""
+    crs, s2c = _geom(params)[:2]
+    rays = np.ascontiguousarray(rays, dtype=np.int32)
+    n = len(rays)
+    off = np.zeros(n + 1, dtype=np.int64)
+    atn = np.zeros(n)
+    t12 = np.zeros((n, 2))
+    rc = lib().orc_tx_lists(_p(crs), _p(s2c), _p(rays), n, _p(off), None, None, None, _p(atn),
+                            _p(t12))
+    assert rc == 0
+    m = int(off[-1])
+    col = np.zeros(m, dtype=np.int32)
+    ts = np.zeros(m)
+    ln = np.zeros(m)
+    rc = lib().orc_tx_lists(_p(crs), _p(s2c), _p(rays), n, _p(off), _p(col), _p(ts), _p(ln),
+                            None, None)
+    assert rc == 0
+    return off, col, ts, ln, atn, t12
+
+
+def fprj(params, img, rays, att=False, return_terms=False):
+    """FP64 forward projection of device image ``img`` (320,320,127) over the
+    transaxial bins ``rays``; returns (len(rays), nsinos) float64."""
+    crs, s2c, li2rng, li2sn, li2nos, nsn = _geom(params)
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    assert img.size == NVOX
+    rays = np.ascontiguousarray(rays, dtype=np.int32)
+    out = np.zeros((len(rays), nsn))
+    nt = np.zeros(1, dtype=np.int64)
+    rc = lib().orc_fprj(_p(img), _p(out), _p(rays), len(rays), _p(crs), _p(s2c), _p(li2rng),
+                        _p(li2sn), _p(li2nos), len(li2rng), nsn, int(bool(att)), _p(nt))
+    assert rc == 0
+    return (out, int(nt[0])) if return_terms else out
+
+
+def bprj(params, sino, rays, nthreads=0):
+    """FP64 back projection of ``sino`` (len(rays), nsinos) over ``rays``;
+    returns the device image (320,320,127) float64."""
+    crs, s2c, li2rng, li2sn, li2nos, nsn = _geom(params)
+    rays = np.ascontiguousarray(rays, dtype=np.int32)
+    sino = np.ascontiguousarray(sino, dtype=np.float64)
+    assert sino.shape == (len(rays), nsn)
+    img = np.zeros((320, 320, NZ))
+    rc = lib().orc_bprj(_p(img), _p(sino), _p(rays), len(rays), _p(crs), _p(s2c), _p(li2rng),
+                        _p(li2sn), len(li2rng), nsn, int(nthreads))
+    assert rc == 0
+    return img
+
+
+def osem(params, prompts, rho, sens, mask, subsets, img0, niter, eps=1e-6):
+    """FP64 OSEM following prj/src/recon.cu:459-497: per subset
+    e = FP(x) + rho; e = y / e (0 where |e| < eps); b = BP(e);
+    b = b / sens (0 where |sens| < eps); x = mask ? x*b : 0.
+    prompts/rho: (Naw, nsn) indexed by transaxial bin; sens: (nsub, 320,320,127);
+    subsets: list of int arrays of transaxial bins."""
+    x = np.array(img0, dtype=np.float64)
+    for _ in range(niter):
+        for s, rows in enumerate(subsets):
+            e = fprj(params, x.astype(np.float32), rows) + rho[rows]
+            y = prompts[rows].astype(np.float64)
+            ratio = np.where(np.abs(e) < eps, 0.0, y / np.where(np.abs(e) < eps, 1.0, e))
+            b = bprj(params, ratio, rows)
+            sv = sens[s]
+            b = np.where(np.abs(sv) < eps, 0.0, b / np.where(np.abs(sv) < eps, 1.0, sv))
+            x = np.where(mask, x * b, 0.0)
+    return x
