@@ -1,0 +1,198 @@
+#!/usr/bin/env python
+"""Benchmark: FP+BP pairs/s of a full mMR span-11 OSEM reconstruction on one
+MI355X (the workload of the reference's osemone -> petprj.osem path).
+
+One step = one complete reconstruction from data already resident in HBM:
+corrections rho = (r+s)/(a n) and a n, the 14 subset sensitivity images
+BP_s(a n), and 4 iterations x 14 subsets of FP (fused with y/(FP+rho)), BP and
+the masked multiplicative update. value = 56 FP+BP pairs per step / step time
+(the 14 sensitivity back projections are inside the timed step but not
+counted as pairs).
+
+Data (synthetic, seeded): device-layout phantom = water ellipsoid (activity 1)
+plus 20 spheres r U(0.5, 2.5) cm activity U(0.5, 4); mu-map water 0.096/cm
+with a bone shell 0.15/cm; prompts = Poisson(scale (a n FP(phantom)) + randoms
++ scatter) at 3e8 expected counts; norm U(0.8, 1.2); randoms flat 12 % and
+scatter smooth 25 % of the trues.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FP+BP pairs/s & % HBM peak (mMR span-11); batched bootstrap recons/s vs plain"
+HBM_PEAK_GBS = 8000.0
+NITER, NSUB = 4, 14
+
+
+_T0 = time.time()
+
+
+def log(msg):
+    print(f'[bench {time.time() - _T0:7.1f}s] {msg}', file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=5)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    ap.add_argument('--cpu-seconds', type=float, default=12.0,
+                    help='approximate CPU work of the baseline sample')
+    ap.add_argument('--copy-calibration', type=int, default=0, metavar='BYTES',
+                    help='only run a plain device copy of BYTES (rocprof counter scaling)')
+    return ap.parse_args()
+
+
+def traffic_per_launch(kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), read from
+    profiles/pmc_<kernel>_osem4x14.json when that summary exists."""
+    path = os.path.join(ROOT, 'profiles', f'pmc_{kernel}_osem4x14.json')
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get('hbm_bytes_per_launch')
+
+
+def cpu_baseline(params, budget_s):
+    """FP64 oracle FP+BP over subset 0 on the host cores, repeated on growing
+    ray samples until about budget_s seconds of CPU work; pairs/s of a full
+    subset."""
+    import numpy as np
+
+    import nipet_amd as nip
+    from oracle import oracle as O
+    sub = nip.get_subsets14(0, params)[0]
+    rng = np.random.default_rng(0)
+    img = rng.random((320, 320, 127), dtype=np.float32)
+    rays_done, dt = 0, 0.0
+    n = 512
+    while dt < budget_s:
+        rows = np.sort(rng.choice(sub, min(n, len(sub)), replace=False)).astype(np.int32)
+        t0 = time.perf_counter()
+        y = O.fprj(params, img, rows)
+        O.bprj(params, y, rows)
+        dt += time.perf_counter() - t0
+        rays_done += len(rows)
+        n *= 2
+    pairs_s = rays_done / (dt * len(sub))
+    return {'value': pairs_s, 'unit': 'pairs/s', 'cores': int(O.lib().orc_max_threads()),
+            'kind': 'port',
+            'sample': f'FP64 oracle FP+BP (emission) over {rays_done} subset-0 rays in '
+                      f'{dt:.1f} s, scaled to one full-subset pair ({len(sub)} rays)'}
+
+
+def copy_calibration(nbytes):
+    """Plain device copy of nbytes (torch copy_), for scaling the rocprofv3
+    FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    n = nbytes // 4
+    src = torch.rand(n, device='cuda')
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({'copy_bytes_read': 4 * n, 'copy_bytes_written': 4 * n, 'launches': 3}),
+          flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f'bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported '
+              '(the multi-GPU sharding is a design note, not an implementation)',
+              file=sys.stderr)
+        sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    import numpy as np
+    import torch
+
+    import nipet_amd as nip
+    from nipet_amd import _lib
+    from nipet_amd.geom import count_terms, get_geom
+    from nipet_amd.prj import mmrsim
+    from nipet_amd.prj.osem import osem_prepare, osem_run
+    assert torch.cuda.is_available(), 'bench.py needs a HIP device'
+    params = nip.get_mmrparams()
+    Cnt = params['Cnt']
+    log('building geometry')
+    geom = get_geom(params)
+    subsets = [nip.get_subsets14(k, params)[0] for k in range(NSUB)]
+    subsets_dev = [torch.from_numpy(s).cuda() for s in subsets]
+    mask = nip.get_cylinder(Cnt, rad=29., unival=1, gpu_dim=True) > 0.9
+    mask_dev = torch.from_numpy(mask.astype(np.uint8)).cuda()
+    log('generating synthetic data')
+    d = mmrsim.osem_inputs(params, counts=3e8)
+    torch.cuda.synchronize()
+    log('data ready')
+
+    def step():
+        st = osem_prepare(params, d['prompts'], d['attn'], d['norm'], d['randoms'], d['scatter'],
+                          subsets=subsets_dev, mask=mask_dev)
+        return osem_run(st, NITER, check_counts=False)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    log('warmup done')
+    with _lib.KernelTimer() as kt:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            img = step()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+    ms_step = 1e3 * dt / args.steps
+    log(f'timed {args.steps} steps: {ms_step:.1f} ms/step')
+    pairs = NITER * NSUB
+    value = pairs * args.steps / dt
+
+    # byte model of the dominant kernel (BP): B = 8 I + 4 M per launch
+    tx = geom.tx_lists()
+    terms = np.array([count_terms(params, s, tx).sum() for s in subsets], dtype=np.float64)
+    M = np.array([len(s) * geom.nsn for s in subsets], dtype=np.float64)
+    b_bp = float(np.mean(8 * terms + 4 * M))
+    b_fp = float(np.mean(4 * terms + 4 * M))
+    t_bp = kt.mean_ms(_lib.PROF_BPRJ) * 1e-3
+    t_fp = kt.mean_ms(_lib.PROF_FPRJ) * 1e-3
+    dominant, b_dom, t_dom = ('k_bprj', b_bp, t_bp) if t_bp >= t_fp else ('k_fprj', b_fp, t_fp)
+    achieved = b_dom / t_dom / 1e9
+    roofline = {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS,
+                'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
+                'traffic': traffic_per_launch(dominant), 'kernel': dominant,
+                'bytes_model': 'BP 8I+4M, FP 4I+4M per subset launch',
+                'I_per_subset': float(terms.mean()),
+                'fp_ms': round(t_fp * 1e3, 4), 'bp_ms': round(t_bp * 1e3, 4),
+                'fp_GBs': round(b_fp / t_fp / 1e9, 1), 'bp_GBs': round(b_bp / t_bp / 1e9, 1),
+                'update_ms': round(kt.mean_ms(_lib.PROF_OSEM_UPDATE), 4)}
+    out = {
+        'metric': METRIC, 'value': round(value, 3), 'unit': 'pairs/s', 'n_gpus': 1,
+        'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': round(ms_step, 3),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32',
+        'data': 'synthetic (seeded phantom, mu-map, Poisson prompts at 3e8 counts)',
+        'config': {'workload': 'full OSEM 4 iterations x 14 subsets, mMR span-11, '
+                               '4894 bins x 837 sinograms per subset, image 320x320x127',
+                   'iterations': NITER, 'subsets': NSUB, 'counts': 3e8,
+                   'recons_per_s': round(1e3 / ms_step, 4),
+                   'image_sum': float(img.sum(dtype=torch.float64))},
+        'roofline': roofline,
+    }
+    log('byte model done')
+    if not args.no_cpu:
+        log('CPU baseline')
+        out['cpu_baseline'] = cpu_baseline(params, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -118,8 +118,19 @@ int nipet_osem_subset(const NipetGeom *g, float *img, const uint16_t *prompts, c
                       const float *sens, const uint8_t *mask, const int32_t *subs, int nprj,
                       float *work_sino, float *work_img, void *stream);
 
-/* Number of list-mode words per chunk that nipet_lm_hist accepts in one call. */
-#define NIPET_LM_MAXWORDS (1u << 30)
+/* Kernel timing, the counterpart of the reference's cudaEvent timers printed
+ * at LOG <= LOGDEBUG (prj/src/prjf.cu:274-317, prj/src/prjb.cu:246-307,
+ * lm/src/hst.cu:452-501). When enabled, every launch is bracketed by HIP
+ * events on its stream; nipet_prof_read resolves them (synchronising) and
+ * returns the accumulated milliseconds and launch count of one kernel class. */
+#define NIPET_PROF_FPRJ 0
+#define NIPET_PROF_BPRJ 1
+#define NIPET_PROF_OSEM_UPDATE 2
+#define NIPET_PROF_FILL 3
+#define NIPET_PROF_HIST 4
+int nipet_prof_enable(int on);
+int nipet_prof_reset(void);
+int nipet_prof_read(int kernel, double *total_ms, int64_t *count);
 
 const char *nipet_last_error(void);
 

@@ -15,11 +15,7 @@ LIBPATH = os.path.join(_HERE, 'libnipet_hip.so')
 _lock = threading.Lock()
 _lib = None
 
-# every symbol include/nipet_hip.h declares
-EXPORTS = (
-    'nipet_geom_create', 'nipet_geom_destroy', 'nipet_geom_info', 'nipet_tx_lists', 'nipet_fprj',
-    'nipet_bprj', 'nipet_osem_corr', 'nipet_osem_subset', 'nipet_lm_hist', 'nipet_lm_tags',
-    'nipet_philox4x32', 'nipet_last_error', 'nipet_version')
+PROF_FPRJ, PROF_BPRJ, PROF_OSEM_UPDATE, PROF_FILL, PROF_HIST = range(5)
 
 
 class NipetError(RuntimeError):
@@ -56,6 +52,9 @@ def lib():
             'nipet_remgaps': ([P, P, I, P, I, I, I, P], I),
             'nipet_putgaps': ([P, P, P, I, I, I, P], I),
             'nipet_osem_subset': ([P, P, P, P, P, P, P, I, P, P, P], I),
+            'nipet_prof_enable': ([I], I),
+            'nipet_prof_reset': ([], I),
+            'nipet_prof_read': ([I, P, P], I),
             'nipet_last_error': ([], ct.c_char_p),
             'nipet_version': ([], ct.c_char_p),
         }
@@ -85,3 +84,28 @@ def ptr(t):
 def stream_handle(device=None):
     import torch
     return torch.cuda.current_stream(device).cuda_stream
+
+
+class KernelTimer:
+    """Context manager: per-kernel-class HIP-event timing inside the library.
+    ``times[k] = (total_ms, launches)`` after exit."""
+
+    def __enter__(self):
+        L = lib()
+        check(L.nipet_prof_reset(), 'nipet_prof_reset')
+        check(L.nipet_prof_enable(1), 'nipet_prof_enable')
+        return self
+
+    def __exit__(self, *exc):
+        L = lib()
+        check(L.nipet_prof_enable(0), 'nipet_prof_enable')
+        self.times = {}
+        for k in range(5):
+            ms, n = ct.c_double(), ct.c_int64()
+            check(L.nipet_prof_read(k, ct.byref(ms), ct.byref(n)), 'nipet_prof_read')
+            self.times[k] = (ms.value, n.value)
+        return False
+
+    def mean_ms(self, k):
+        ms, n = self.times[k]
+        return ms / n if n else float('nan')

@@ -43,6 +43,12 @@ struct Geom {
 void set_error(const char *fmt, ...);
 int hip_fail(hipError_t e, const char *what);
 
+// Opt-in kernel timing (nipet_prof_*): HIP events recorded around launches on
+// the launch stream, resolved when read.
+enum ProfKernel { PK_FPRJ = 0, PK_BPRJ, PK_OSEM_UPDATE, PK_FILL, PK_HIST, PK_OTHER, PK_COUNT };
+void prof_begin(int kernel, hipStream_t st);
+void prof_end(int kernel, hipStream_t st);
+
 }  // namespace nipet
 
 struct NipetGeom : nipet::Geom {};

@@ -20,6 +20,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "nipet_common.h"
@@ -38,6 +39,60 @@ void set_error(const char *fmt, ...) {
 int hip_fail(hipError_t e, const char *what) {
   set_error("HIP error %d (%s) in %s", (int)e, hipGetErrorString(e), what);
   return (e == hipErrorOutOfMemory) ? NIPET_ENOMEM : NIPET_EHIP;
+}
+
+// ---- kernel timing ----
+struct ProfRec {
+  int kernel;
+  hipEvent_t e0, e1;
+};
+static std::mutex g_prof_mu;
+static bool g_prof_on = false;
+static std::vector<ProfRec> g_prof_pending;
+static std::vector<hipEvent_t> g_prof_pool;
+static double g_prof_ms[PK_COUNT];
+static int64_t g_prof_n[PK_COUNT];
+static thread_local hipEvent_t g_prof_open[PK_COUNT];
+
+static hipEvent_t prof_event() {
+  hipEvent_t e = nullptr;
+  if (!g_prof_pool.empty()) {
+    e = g_prof_pool.back();
+    g_prof_pool.pop_back();
+  } else if (hipEventCreate(&e) != hipSuccess) {
+    e = nullptr;
+  }
+  return e;
+}
+
+void prof_begin(int k, hipStream_t st) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  if (!g_prof_on) return;
+  hipEvent_t e = prof_event();
+  if (e && hipEventRecord(e, st) == hipSuccess) g_prof_open[k] = e;
+}
+
+void prof_end(int k, hipStream_t st) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  if (!g_prof_on || !g_prof_open[k]) return;
+  hipEvent_t e = prof_event();
+  if (e && hipEventRecord(e, st) == hipSuccess)
+    g_prof_pending.push_back(ProfRec{k, g_prof_open[k], e});
+  g_prof_open[k] = nullptr;
+}
+
+static int prof_resolve() {
+  for (auto &r : g_prof_pending) {
+    NIPET_HIP(hipEventSynchronize(r.e1));
+    float ms = 0.f;
+    NIPET_HIP(hipEventElapsedTime(&ms, r.e0, r.e1));
+    g_prof_ms[r.kernel] += ms;
+    g_prof_n[r.kernel] += 1;
+    g_prof_pool.push_back(r.e0);
+    g_prof_pool.push_back(r.e1);
+  }
+  g_prof_pending.clear();
+  return NIPET_OK;
 }
 
 #pragma clang fp contract(off)
@@ -188,6 +243,34 @@ extern "C" {
 const char *nipet_last_error(void) { return g_err; }
 
 const char *nipet_version(void) { return "nipet_amd 0.1 (gfx950)"; }
+
+int nipet_prof_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof_on = on != 0;
+  return NIPET_OK;
+}
+
+int nipet_prof_reset(void) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  int rc = prof_resolve();
+  for (int k = 0; k < PK_COUNT; k++) {
+    g_prof_ms[k] = 0.0;
+    g_prof_n[k] = 0;
+  }
+  return rc;
+}
+
+int nipet_prof_read(int kernel, double *total_ms, int64_t *count) {
+  if (kernel < 0 || kernel >= PK_COUNT || !total_ms || !count) {
+    set_error("nipet_prof_read: bad argument");
+    return NIPET_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  int rc = prof_resolve();
+  *total_ms = g_prof_ms[kernel];
+  *count = g_prof_n[kernel];
+  return rc;
+}
 
 static void geom_free(Geom *g) {
   hipFree(g->off);

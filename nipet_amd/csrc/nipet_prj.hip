@@ -492,10 +492,12 @@ int nipet_fprj(const NipetGeom *g, float *sino, const float *img, const int32_t 
     return NIPET_EINVAL;
   }
   hipStream_t st = (hipStream_t)stream;
+  prof_begin(PK_FPRJ, st);
   if (att)
     k_fprj<1><<<nprj, PRJ_THREADS, 0, st>>>(view(g), img, sino, subs, nullptr, nullptr);
   else
     k_fprj<0><<<nprj, PRJ_THREADS, 0, st>>>(view(g), img, sino, subs, nullptr, nullptr);
+  prof_end(PK_FPRJ, st);
   NIPET_HIP(hipGetLastError());
   return NIPET_OK;
 }
@@ -510,8 +512,12 @@ int nipet_bprj(const NipetGeom *g, float *img, const float *sino, const int32_t 
     return NIPET_EINVAL;
   }
   hipStream_t st = (hipStream_t)stream;
+  prof_begin(PK_FILL, st);
   k_fill<<<ew_blocks(NVOX), 256, 0, st>>>(img, 0.f, NVOX);
+  prof_end(PK_FILL, st);
+  prof_begin(PK_BPRJ, st);
   k_bprj<<<nprj, PRJ_THREADS, 0, st>>>(view(g), img, sino, subs);
+  prof_end(PK_BPRJ, st);
   NIPET_HIP(hipGetLastError());
   return NIPET_OK;
 }
@@ -574,10 +580,18 @@ int nipet_osem_subset(const NipetGeom *g, float *img, const uint16_t *prompts, c
     return NIPET_EINVAL;
   }
   hipStream_t st = (hipStream_t)stream;
+  prof_begin(PK_FPRJ, st);
   k_fprj<2><<<nprj, PRJ_THREADS, 0, st>>>(view(g), img, work_sino, subs, prompts, rho);
+  prof_end(PK_FPRJ, st);
+  prof_begin(PK_FILL, st);
   k_fill<<<ew_blocks(NVOX), 256, 0, st>>>(work_img, 0.f, NVOX);
+  prof_end(PK_FILL, st);
+  prof_begin(PK_BPRJ, st);
   k_bprj<<<nprj, PRJ_THREADS, 0, st>>>(view(g), work_img, work_sino, subs);
+  prof_end(PK_BPRJ, st);
+  prof_begin(PK_OSEM_UPDATE, st);
   k_osem_update<<<ew_blocks(NVOX), 256, 0, st>>>(img, work_img, sens, mask, NVOX);
+  prof_end(PK_OSEM_UPDATE, st);
   NIPET_HIP(hipGetLastError());
   return NIPET_OK;
 }

@@ -89,3 +89,34 @@ def get_geom(scanner_params, device=None):
         g = DeviceGeom(scanner_params, dev)
         _CACHE[key] = g
     return g
+
+
+def count_terms(scanner_params, rays, tx=None):
+    """Number I of (span-1 LOR, voxel) weight terms of the projector over the
+    transaxial bins ``rays`` (z-plane splits count as separate terms): per
+    bin, every LOR walks the bin's nseg segments, and an oblique LOR with ring
+    difference d crosses floor(0.5 + 2d e/atn) - floor(0.5 + 2d s/atn) z-planes
+    between the first segment start s and the last segment end e (the same for
+    every r0 and for the mirror). ``tx``: optional (offsets, col, t, len) lists.
+    Returns an int64 array per bin."""
+    Cnt, txLUT, axLUT = scanner_params['Cnt'], scanner_params['txLUT'], scanner_params['axLUT']
+    rays = np.asarray(rays, dtype=np.int64)
+    if tx is None:
+        tx = get_geom(scanner_params).tx_lists()
+    off, _, t, ln = tx
+    crs = txLUT['crs'].astype(np.float64)
+    c = txLUT['s2c'][rays].astype(np.int64)
+    p = 0.5 * (crs[c[:, 0], :2] + crs[c[:, 0], 2:])
+    q = 0.5 * (crs[c[:, 1], :2] + crs[c[:, 1], 2:])
+    atn = np.hypot(*(q - p).T)
+    b, e = off[rays], off[rays + 1]
+    nseg = (e - b).astype(np.int64)
+    s0 = t[b].astype(np.float64)
+    e1 = t[e - 1].astype(np.float64) + ln[e - 1].astype(np.float64)
+    r = axLUT['li2rno'].astype(np.int64)
+    dcount = np.bincount(r[:, 1] - r[:, 0])          # LOR pairs per ring difference
+    d = np.arange(1, len(dcount))
+    cross = (np.floor(0.5 + 2 * d[None, :] * e1[:, None] / atn[:, None]) -
+             np.floor(0.5 + 2 * d[None, :] * s0[:, None] / atn[:, None])).astype(np.int64)
+    nlor = dcount[0] + 2 * dcount[1:].sum()
+    return nseg * nlor + 2 * (cross * dcount[1:][None, :]).sum(1)

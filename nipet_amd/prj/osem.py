@@ -1,0 +1,122 @@
+"""Device-resident OSEM (the ``petprj.osem`` loop, ``prj/src/recon.cu:318-521``,
+driven by ``osemone``, ``prj/mmrrec.py:279-389``).
+
+Differences in structure (not in arithmetic) from the reference:
+* prompts, corrections, sensitivity images, mask and image are uploaded once
+  and stay in HBM for all iterations (the reference re-uploads about 1.8 GB per
+  iteration, ``prj/src/recon.cu:333-419``);
+* the transaxial intersection lists are built once per geometry;
+* per subset, three launches: FP fused with ``y / (FP + rho)``, the gather BP,
+  and the fused ``img * BP / sens`` masked update (``nipet_osem_subset``).
+"""
+import numpy as np
+
+from .. import _lib
+from ..geom import get_geom
+
+__all__ = ['OsemState', 'osem_prepare', 'osem_run', 'sensitivity']
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _dev(x, dtype):
+    torch = _torch()
+    if isinstance(x, torch.Tensor):
+        return x.to('cuda', dtype=dtype).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(x)).to('cuda', dtype=dtype)
+
+
+class OsemState:
+    """Everything one OSEM reconstruction keeps on the device."""
+
+    def __init__(self, geom, subsets, prompts, rho, an, sens, mask):
+        torch = _torch()
+        self.geom = geom
+        self.subsets = subsets
+        self.nprj = max(len(s) for s in subsets)
+        self.prompts, self.rho, self.an, self.sens, self.mask = prompts, rho, an, sens, mask
+        self.work_sino = torch.empty((self.nprj, geom.nsn), dtype=torch.float32, device='cuda')
+        self.work_img = torch.empty((320, 320, 127), dtype=torch.float32, device='cuda')
+
+
+def sensitivity(geom, an, subs):
+    """BP_subset(a*n) (``prj/mmrrec.py:291-298``) for one subset (device tensors)."""
+    torch = _torch()
+    rows = an.index_select(0, subs.long())
+    out = torch.empty((320, 320, 127), dtype=torch.float32, device='cuda')
+    _lib.check(_lib.lib().nipet_bprj(geom.handle, _lib.ptr(out), _lib.ptr(rows), _lib.ptr(subs),
+                                     len(subs), _lib.stream_handle()), 'nipet_bprj')
+    return out
+
+
+def osem_prepare(scanner_params, psng, asng, nsng, rsng=None, ssng=None, mask_radius=29.,
+                 subsets=None, mask=None):
+    """Upload the data and compute rho = (r+s)/(a n), a n and the per-subset
+    sensitivity images. Sinograms are device-shaped (Naw, nsn); prompts are
+    converted to uint16 as in ``petprj.osem`` (``prj/src/prj_module.cu:649-651``)."""
+    torch = _torch()
+    from ..img.mmrimg import get_cylinder
+    from .mmrrec import get_subsets14
+    geom = get_geom(scanner_params)
+    Cnt = scanner_params['Cnt']
+    shape = (geom.naw, geom.nsn)
+    if isinstance(psng, torch.Tensor):
+        prompts = psng.to('cuda')
+        if prompts.dtype != torch.uint16:
+            prompts = prompts.to(torch.int32).clamp(0, 65535).to(torch.uint16)
+        prompts = prompts.contiguous()
+    else:
+        # the reference hands petprj.osem a uint16 array (a cast truncates)
+        prompts = _dev(np.asarray(psng).astype(np.uint16), torch.uint16)
+    if tuple(prompts.shape) != shape:
+        raise ValueError(f'prompts must be device-shaped {shape}')
+    a = _dev(asng, torch.float32)
+    n = _dev(nsng, torch.float32)
+    r = None if rsng is None else _dev(rsng, torch.float32)
+    s = None if ssng is None else _dev(ssng, torch.float32)
+    for x in (a, n, r, s):
+        if x is not None and tuple(x.shape) != shape:
+            raise ValueError(f'corrections must be device-shaped {shape}')
+    rho = torch.empty(shape, dtype=torch.float32, device='cuda')
+    an = torch.empty(shape, dtype=torch.float32, device='cuda')
+    _lib.check(_lib.lib().nipet_osem_corr(_lib.ptr(rho), _lib.ptr(an), _lib.ptr(a), _lib.ptr(n),
+                                          _lib.ptr(r), _lib.ptr(s), rho.numel(),
+                                          _lib.stream_handle()), 'nipet_osem_corr')
+    if subsets is None:
+        subsets = [get_subsets14(k, scanner_params)[0] for k in range(14)]
+    subs = [x.to('cuda', torch.int32).contiguous() if isinstance(x, torch.Tensor) else
+            _dev(np.asarray(x, np.int32), torch.int32) for x in subsets]
+    sens = torch.stack([sensitivity(geom, an, x) for x in subs])
+    if mask is None:
+        mask = get_cylinder(Cnt, rad=mask_radius, xo=0, yo=0, unival=1, gpu_dim=True) > 0.9
+    msk = _dev(mask if isinstance(mask, torch.Tensor) else np.asarray(mask, np.uint8),
+               torch.uint8)
+    return OsemState(geom, subs, prompts, rho, an, sens, msk)
+
+
+def osem_iteration(state, img):
+    """One OSEM iteration (all subsets) in place on the device image ``img``."""
+    st = _lib.stream_handle()
+    L = _lib.lib()
+    for k, subs in enumerate(state.subsets):
+        _lib.check(L.nipet_osem_subset(state.geom.handle, _lib.ptr(img), _lib.ptr(state.prompts),
+                                       _lib.ptr(state.rho), _lib.ptr(state.sens[k]),
+                                       _lib.ptr(state.mask), _lib.ptr(subs), len(subs),
+                                       _lib.ptr(state.work_sino), _lib.ptr(state.work_img), st),
+                   'nipet_osem_subset')
+
+
+def osem_run(state, niter, img0=None, check_counts=True):
+    """``niter`` OSEM iterations from ``img0`` (ones by default, ``prj/mmrrec.py:306``).
+    Stops early, like ``prj/mmrrec.py:357-361``, when the image sum drops below 0.1."""
+    torch = _torch()
+    img = torch.ones((320, 320, 127), dtype=torch.float32, device='cuda') if img0 is None \
+        else _dev(img0, torch.float32).clone()
+    for _ in range(niter):
+        osem_iteration(state, img)
+        if check_counts and float(torch.nansum(img)) < 0.1:
+            break
+    return img

@@ -232,9 +232,10 @@ static void lor_setup(const float *li2rng, int li, double atn, lor_t *L) {
  * li2sn (nli,2) maps LOR a / mirror b to the output sinogram; li2nos (nli)
  * is the number of span-1 LORs per output bin. nterms (optional) receives
  * the number of (LOR, voxel) pieces. */
-int orc_fprj(const float *img, double *sino, const int32_t *rays, int nray, const float *crs,
-             const int16_t *s2c, const float *li2rng, const int16_t *li2sn, const int8_t *li2nos,
-             int nli, int nsn, int att, int64_t *nterms) {
+static int fprj_impl(const float *imgf, const double *imgd, double *sino, const int32_t *rays,
+                     int nray, const float *crs, const int16_t *s2c, const float *li2rng,
+                     const int16_t *li2sn, const int8_t *li2nos, int nli, int nsn, int att,
+                     int64_t *nterms) {
   int64_t tot = 0;
   int err = 0;
 #pragma omp parallel for schedule(dynamic, 4) reduction(+ : tot)
@@ -255,10 +256,12 @@ int orc_fprj(const float *img, double *sino, const int32_t *rays, int nray, cons
       lor_setup(li2rng, li, r.atn, &L);
       double acc = 0, accb = 0;
       int64_t np = 0;
-#define ACC_A(w, c, wt) (acc += (wt) * (double)img[(size_t)(c) * NZ + (w)],                 \
-                         accb += (wt) * (double)img[(size_t)(c) * NZ + (L.wsum - (w))])
+#define IMG(k) (imgd ? imgd[k] : (double)imgf[k])
+#define ACC_A(w, c, wt) (acc += (wt) * IMG((size_t)(c) * NZ + (w)),                          \
+                         accb += (wt) * IMG((size_t)(c) * NZ + (L.wsum - (w))))
       WALK_LOR(L, nseg, col, ts, len, ACC_A, np);
 #undef ACC_A
+#undef IMG
       if (L.direct) {
         tot += np;
         if (att)
@@ -279,6 +282,21 @@ int orc_fprj(const float *img, double *sino, const int32_t *rays, int nray, cons
   }
   if (nterms) *nterms = tot;
   return err ? -1 : 0;
+}
+
+int orc_fprj(const float *img, double *sino, const int32_t *rays, int nray, const float *crs,
+             const int16_t *s2c, const float *li2rng, const int16_t *li2sn, const int8_t *li2nos,
+             int nli, int nsn, int att, int64_t *nterms) {
+  return fprj_impl(img, NULL, sino, rays, nray, crs, s2c, li2rng, li2sn, li2nos, nli, nsn, att,
+                   nterms);
+}
+
+/* Same, for a double-precision image. */
+int orc_fprj_d(const double *img, double *sino, const int32_t *rays, int nray, const float *crs,
+               const int16_t *s2c, const float *li2rng, const int16_t *li2sn,
+               const int8_t *li2nos, int nli, int nsn, int att, int64_t *nterms) {
+  return fprj_impl(NULL, img, sino, rays, nray, crs, s2c, li2rng, li2sn, li2nos, nli, nsn, att,
+                   nterms);
 }
 
 /* Back projection (adjoint of the emission forward projection) of

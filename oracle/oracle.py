@@ -30,6 +30,8 @@ def lib():
         P = ct.c_void_p
         L.orc_tx_lists.argtypes = [P, P, P, ct.c_int, P, P, P, P, P, P]
         L.orc_fprj.argtypes = [P, P, P, ct.c_int, P, P, P, P, P, ct.c_int, ct.c_int, ct.c_int, P]
+        L.orc_fprj_d.argtypes = L.orc_fprj.argtypes
+        L.orc_fprj_d.restype = ct.c_int
         L.orc_bprj.argtypes = [P, P, P, ct.c_int, P, P, P, P, ct.c_int, ct.c_int, ct.c_int]
         for f in (L.orc_tx_lists, L.orc_fprj, L.orc_bprj):
             f.restype = ct.c_int
@@ -78,16 +80,19 @@ def tx_lists(params, rays):
 
 
 def fprj(params, img, rays, att=False, return_terms=False):
-    """FP64 forward projection of device image ``img`` (320,320,127) over the
+    """FP64 forward projection of device image ``img`` (320,320,127; float32 or
+    float64) over the
     transaxial bins ``rays``; returns (len(rays), nsinos) float64."""
     crs, s2c, li2rng, li2sn, li2nos, nsn = _geom(params)
-    img = np.ascontiguousarray(img, dtype=np.float32)
+    dbl = np.asarray(img).dtype == np.float64
+    img = np.ascontiguousarray(img, dtype=np.float64 if dbl else np.float32)
     assert img.size == NVOX
     rays = np.ascontiguousarray(rays, dtype=np.int32)
     out = np.zeros((len(rays), nsn))
     nt = np.zeros(1, dtype=np.int64)
-    rc = lib().orc_fprj(_p(img), _p(out), _p(rays), len(rays), _p(crs), _p(s2c), _p(li2rng),
-                        _p(li2sn), _p(li2nos), len(li2rng), nsn, int(bool(att)), _p(nt))
+    f = lib().orc_fprj_d if dbl else lib().orc_fprj
+    rc = f(_p(img), _p(out), _p(rays), len(rays), _p(crs), _p(s2c), _p(li2rng),
+           _p(li2sn), _p(li2nos), len(li2rng), nsn, int(bool(att)), _p(nt))
     assert rc == 0
     return (out, int(nt[0])) if return_terms else out
 
@@ -115,7 +120,7 @@ def osem(params, prompts, rho, sens, mask, subsets, img0, niter, eps=1e-6):
     x = np.array(img0, dtype=np.float64)
     for _ in range(niter):
         for s, rows in enumerate(subsets):
-            e = fprj(params, x.astype(np.float32), rows) + rho[rows]
+            e = fprj(params, x, rows) + rho[rows]
             y = prompts[rows].astype(np.float64)
             ratio = np.where(np.abs(e) < eps, 0.0, y / np.where(np.abs(e) < eps, 1.0, e))
             b = bprj(params, ratio, rows)

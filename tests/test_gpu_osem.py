@@ -1,0 +1,121 @@
+"""GPU OSEM parity against the FP64 oracle OSEM (oracle/oracle.py::osem,
+following prj/src/recon.cu:459-497) and MLEM invariants.
+
+Bars: images within 1e-4 relative L2 of the oracle; MLEM count preservation
+sum_S A_S x' = sum_S y to 1e-5 relative.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope='module')
+def torch():
+    import torch
+    assert torch.cuda.is_available(), 'GPU tests need a HIP device'
+    return torch
+
+
+def _mini(params, subsets, step=20):
+    return [subsets[k][::step].astype(np.int32) for k in (0, 3, 7, 10)]
+
+
+def _data(params, rows_all, seed=0):
+    naw = params['txLUT']['Naw']
+    rng = np.random.default_rng(seed)
+    y = np.zeros((naw, 837), np.uint16)
+    a = np.ones((naw, 837), np.float32)
+    n = np.ones((naw, 837), np.float32)
+    r = np.zeros((naw, 837), np.float32)
+    y[rows_all] = rng.poisson(6.0, size=(len(rows_all), 837))
+    a[rows_all] = rng.uniform(0.3, 1.0, size=(len(rows_all), 837))
+    n[rows_all] = rng.uniform(0.8, 1.2, size=(len(rows_all), 837))
+    r[rows_all] = rng.uniform(0.0, 0.5, size=(len(rows_all), 837))
+    a[rows_all[:5], :3] = 0.0      # exercise the |a n| < eps guard
+    return y, a, n, r
+
+
+def test_osem_corr_guards(torch, params):
+    from nipet_amd import _lib
+    a = torch.tensor([0.5, 1e-7, 0.0, 2.0], device='cuda')
+    n = torch.tensor([2.0, 1.0, 1.0, -0.25], device='cuda')
+    r = torch.tensor([1.0, 3.0, 3.0, 1.0], device='cuda')
+    s = torch.tensor([0.5, 0.0, 1.0, 0.0], device='cuda')
+    rho, an = torch.empty_like(a), torch.empty_like(a)
+    _lib.check(_lib.lib().nipet_osem_corr(rho.data_ptr(), an.data_ptr(), a.data_ptr(),
+                                          n.data_ptr(), r.data_ptr(), s.data_ptr(), 4,
+                                          _lib.stream_handle()))
+    torch.cuda.synchronize()
+    assert torch.allclose(an, a * n)
+    assert rho.tolist() == [1.5, 0.0, 0.0, -2.0]
+
+
+def test_osem_mini_vs_oracle(torch, params, subsets):
+    from oracle import oracle as O
+
+    from nipet_amd.prj.osem import osem_prepare, osem_run
+    mini = _mini(params, subsets)
+    rows_all = np.concatenate(mini)
+    y, a, n, r = _data(params, rows_all)
+    mask = np.ones((320, 320, 127), bool)
+    mask[:, :, :3] = False
+    st = osem_prepare(params, y, a, n, r, None, subsets=mini, mask=mask)
+    gpu = osem_run(st, 2).cpu().numpy().astype(np.float64)
+    an = a.astype(np.float64) * n
+    rho = np.where(np.abs(an) < 1e-6, 0.0, r / np.where(np.abs(an) < 1e-6, 1.0, an))
+    sens = np.stack([O.bprj(params, an[s], s) for s in mini])
+    ref = O.osem(params, y, rho, sens, mask, mini, np.ones((320, 320, 127)), 2)
+    rel = np.linalg.norm(gpu - ref) / np.linalg.norm(ref)
+    assert rel < 1e-4, rel
+    assert np.all(gpu[~mask] == 0)
+
+
+def test_mlem_count_preservation(torch, params, subsets):
+    import nipet_amd as nip
+    from nipet_amd.prj.osem import osem_prepare, osem_run
+    sub = subsets[4][::4].astype(np.int32)
+    naw = params['txLUT']['Naw']
+    y = np.zeros((naw, 837), np.uint16)
+    y[sub] = np.random.default_rng(1).poisson(4.0, size=(len(sub), 837))
+    ones = np.ones((naw, 837), np.float32)
+    st = osem_prepare(params, y, ones, ones, None, None, subsets=[sub],
+                      mask=np.ones((320, 320, 127), bool))
+    x1 = osem_run(st, 1)
+    fp = nip.frwd_prj(x1, params, isub=sub, dev_out=True, fullsino_out=False)
+    tot = float(fp.sum(dtype=torch.float64))
+    ysum = float(y[sub].sum(dtype=np.float64))
+    assert abs(tot - ysum) / ysum < 1e-5
+
+
+def test_osem_4x14_full_vs_golden(torch, params, subsets):
+    """Full config: 4 iterations x 14 subsets, span-11, 320x320x127, on the
+    synthetic inputs of tests/golden/osem_data.py; the oracle image (FP64)
+    is stored as 200,000 sampled voxels in osem4x14_sample.npz."""
+    import sys
+    sys.path.insert(0, os.path.join(HERE, 'golden'))
+    from osem_data import checksum, make_osem_data
+
+    import nipet_amd as nip
+    from nipet_amd.prj.osem import osem_prepare, osem_run
+    fx = os.path.join(HERE, 'golden', 'osem4x14_sample.npz')
+    if not os.path.exists(fx):
+        pytest.skip('golden fixture not generated (tests/golden/make_osem_golden.py)')
+    g = np.load(fx)
+    d = make_osem_data(params)
+    assert tuple(checksum(d['prompts'])) == tuple(int(v) for v in g['checksum']), \
+        'synthetic inputs differ from the ones the fixture was made from'
+    mask = nip.get_cylinder(params['Cnt'], rad=29., unival=1, gpu_dim=True) > 0.9
+    st = osem_prepare(params, d['prompts'], d['attn'], d['norm'], d['randoms'], d['scatter'],
+                      subsets=subsets, mask=mask)
+    img = osem_run(st, int(g['niter'])).cpu().numpy().astype(np.float64).ravel()
+    ref = g['val']
+    got = img[g['idx']]
+    rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert rel < 1e-4, rel
+    assert abs(img.sum() - float(g['total'])) / float(g['total']) < 1e-4
+    assert abs(np.sqrt((img * img).sum()) - float(g['l2'])) / float(g['l2']) < 1e-4

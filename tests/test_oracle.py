@@ -145,3 +145,14 @@ def test_bprj_single_bin(params):
     fp1 = O.fprj(params, np.ones((320, 320, 127), np.float32), rows)
     assert abs(bp.sum() - fp1[0, 400]) < 1e-9 * fp1[0, 400]
     assert np.count_nonzero(bp) > 0
+
+
+def test_term_count_formula(params):
+    # the closed-form term count used by bench.py's byte model equals the
+    # oracle's piece count
+    from nipet_amd.geom import count_terms
+    rays = np.array([0, 777, 12000, 34257, 50000, 68000], np.int32)
+    tx = O.tx_lists(params, np.arange(68516, dtype=np.int32))[:4]
+    ref = [O.fprj(params, np.zeros((320, 320, 127), np.float32), rays[i:i + 1],
+                  return_terms=True)[1] for i in range(len(rays))]
+    assert np.array_equal(count_terms(params, rays, tx), ref)
