@@ -118,6 +118,54 @@ int nipet_osem_subset(const NipetGeom *g, float *img, const uint16_t *prompts, c
                       const float *sens, const uint8_t *mask, const int32_t *subs, int nprj,
                       float *work_sino, float *work_img, void *stream);
 
+/* ---- list-mode histogramming (mmr_lmproc.hist: lm/src/lm_module.cu:174-391,
+ * lmproc lm/src/lmproc.cu:12-230, gpu_hst / hst lm/src/hst.cu:37-528) ----
+ * A session histograms one frame [tstart, tstop) (seconds from the first time
+ * tag toff) of a word stream handed over in consecutive chunks; the time-tag
+ * state and the word index (bootstrap counter) carry over between chunks.
+ * Each event belongs to the most recent preceding time tag. */
+typedef struct NipetLm NipetLm;
+
+typedef struct {
+  int toff;              /* first time tag of the acquisition [ms] (lminfo) */
+  int nitag;             /* seconds covered: ((last tag - toff) + 999) / 1000 */
+  int tstart, tstop;     /* frame [tstart, tstop) in seconds */
+  int span;              /* 11 (837 sinograms), 1 (4084) or 0 (SSRB, 127) */
+  int btp;               /* 0: no bootstrap; 2: parametric (Poisson) bootstrap */
+  double btprt;          /* Poisson mean of the bootstrap weights (Cnt['BTPRT']) */
+  uint64_t seed;         /* Philox4x32-10 key */
+  int nrep;              /* replicates 1..64; nrep > 1 stores [bin][nrep] */
+  const int16_t *sn1_sn11;  /* host [4084] (axLUT['sn1_sn11']) */
+  const int16_t *sn1_ssrb;  /* host [4084] (axLUT['sn1_ssrb']) */
+  const int16_t *sn1_rno;   /* host [4084][2] (axLUT['sn1_rno']) */
+  const int16_t *s2cF;      /* host [86688][2] (txLUT['s2cF']) */
+} NipetLmConfig;
+
+/* Device outputs, zeroed by the caller, accumulated by every chunk. */
+typedef struct {
+  uint32_t *psino, *dsino;  /* prompts / delayeds [nbins][nrep] (nbins = nsinos*86688) */
+  uint32_t *ssr;            /* SSRB prompts [127][252][344] (replicate 0) */
+  uint32_t *phc, *dhc;      /* head curves [nitag][nrep] */
+  uint64_t *zR, *zM;        /* centre-of-mass sums [nitag] (replicate 0) */
+  uint32_t *pvs;            /* projection views [(min(nitag,5400)+3)/4][127][344] */
+  uint32_t *fan;            /* delayed fan-sums [64][504] */
+  uint32_t *bck;            /* singles buckets [2][nitag][224] */
+} NipetLmOut;
+
+int nipet_lm_create(const NipetLmConfig *cfg, NipetLm **out);
+/* Histogram words[0..n) (device int32) on `stream`. */
+int nipet_lm_chunk(NipetLm *h, const int32_t *words, int64_t n, const NipetLmOut *out,
+                   void *stream);
+/* Words processed, words with an address beyond the span-1 range (skipped),
+ * singles reports lost to list overflow. Synchronous. */
+int nipet_lm_status(NipetLm *h, int64_t *words_done, unsigned int *bad_words,
+                    unsigned int *singles_overflow);
+void nipet_lm_destroy(NipetLm *h);
+
+/* Philox4x32-10 on the device for one counter/key (host arrays), for
+ * known-answer checks of the bootstrap generator. Synchronous. */
+int nipet_philox4x32(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
+
 /* Kernel timing, the counterpart of the reference's cudaEvent timers printed
  * at LOG <= LOGDEBUG (prj/src/prjf.cu:274-317, prj/src/prjb.cu:246-307,
  * lm/src/hst.cu:452-501). When enabled, every launch is bracketed by HIP

@@ -22,6 +22,19 @@ class NipetError(RuntimeError):
     pass
 
 
+class NipetLmConfig(ct.Structure):
+    _fields_ = [
+        ('toff', ct.c_int), ('nitag', ct.c_int), ('tstart', ct.c_int), ('tstop', ct.c_int),
+        ('span', ct.c_int), ('btp', ct.c_int), ('btprt', ct.c_double), ('seed', ct.c_uint64),
+        ('nrep', ct.c_int), ('sn1_sn11', ct.c_void_p), ('sn1_ssrb', ct.c_void_p),
+        ('sn1_rno', ct.c_void_p), ('s2cF', ct.c_void_p)]
+
+
+class NipetLmOut(ct.Structure):
+    _fields_ = [(k, ct.c_void_p) for k in
+                ('psino', 'dsino', 'ssr', 'phc', 'dhc', 'zR', 'zM', 'pvs', 'fan', 'bck')]
+
+
 class NipetGeomDesc(ct.Structure):
     _fields_ = [
         ('crs', ct.c_void_p), ('ncrs', ct.c_int), ('s2c', ct.c_void_p), ('naw', ct.c_int),
@@ -52,6 +65,11 @@ def lib():
             'nipet_remgaps': ([P, P, I, P, I, I, I, P], I),
             'nipet_putgaps': ([P, P, P, I, I, I, P], I),
             'nipet_osem_subset': ([P, P, P, P, P, P, P, I, P, P, P], I),
+            'nipet_lm_create': ([ct.POINTER(NipetLmConfig), ct.POINTER(P)], I),
+            'nipet_lm_chunk': ([P, P, I64, ct.POINTER(NipetLmOut), P], I),
+            'nipet_lm_status': ([P, P, P, P], I),
+            'nipet_lm_destroy': ([P], None),
+            'nipet_philox4x32': ([P, P, P], I),
             'nipet_prof_enable': ([I], I),
             'nipet_prof_reset': ([], I),
             'nipet_prof_read': ([I, P, P], I),

@@ -1,0 +1,617 @@
+// List-mode histogramming of Siemens mMR 32-bit words with bootstrap on gfx950.
+//
+// Reference: hst kernel lm/src/hst.cu:37-244, gpu_hst :347-528, lmproc
+// lm/src/lmproc.cu:12-230. Semantics (restated):
+//  - word > 0: event, bit 30 prompt, bits 0-29 span-1 address si*86688 + a*344 + w;
+//    word >> 29 == -4: time tag (ms in bits 0-28); word >> 29 == -3: singles;
+//  - itag = (ms - toff) / 1000 of the most recent preceding time tag (0 before
+//    the first tag); events / singles count for tstart <= itag < tstop;
+//  - bootstrap (btp = 2): weight N ~ Poisson via inversion of a uint32 CDF
+//    against Philox4x32-10(counter = (word index lo, hi, replicate, 0),
+//    key = (seed lo, hi)) output word 0; counted when 0 < N < 32.
+// Layout: with nrep replicates, sinograms and head curves are replicate-
+// innermost ([bin][nrep], [itag][nrep]): one event updates nrep consecutive
+// counters with one coalesced wave atomic. Integer atomics make every count
+// independent of the processing order (bit-exact against a sequential pass).
+//
+// Kernels per chunk of words:
+//  k_lm_tags   last time tag of every block of LM_BLOCK words;
+//  k_lm_carry  parallel "last tag before block b" scan (carry across chunks);
+//  k_lm_hist   events (head curves and centre of mass privatised in LDS over a
+//              window of LM_HW seconds), singles appended to a list;
+//  k_lm_sgl_first / k_lm_sgl_apply  singles buckets in two order-free passes.
+#include <stdlib.h>
+
+#include <cmath>
+#include <vector>
+
+#include "nipet_common.h"
+
+namespace nipet {
+
+constexpr int LM_THREADS = 256;
+constexpr int LM_PER_LANE = 32;
+constexpr int LM_BLOCK = LM_THREADS * LM_PER_LANE;  // words per block
+constexpr int LM_WAVES = LM_THREADS / 64;
+constexpr int LM_HW = 16;                           // LDS head-curve window [s]
+constexpr int LM_MAXREP = 64;
+constexpr int LM_NCDF = 40;
+constexpr int NSBINANG = 86688, NSBINS = 344, NSANGLES = 252, NCRS = 504, NBUCKTS = 224;
+constexpr int NSN1 = 4084, SEG0 = 127, MXNITAG = 5400, VTIME = 2, ITIME = 1000;
+constexpr int32_t TMSK = 0x1fffffff;
+
+__host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    uint64_t p0 = (uint64_t)M0 * c[0], p1 = (uint64_t)M1 * c[2];
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+    c[1] = (uint32_t)p1;
+    c[3] = (uint32_t)p0;
+    c[0] = n0;
+    c[2] = n2;
+    k0 += W0;
+    k1 += W1;
+  }
+}
+
+struct LmDev {
+  const int32_t *words;
+  int64_t n, base;
+  int toff, nitag, tstart, tstop, span, btp, nrep;
+  uint32_t k0, k1;
+  int64_t nbins;
+  const int32_t *lut;   // [NSN1] (sn1_sn11 << 16) | sn1_ssrb
+  const short2 *rno;    // [NSN1] sn1_rno
+  const short2 *s2cF;   // [86688] crystal pair of (a + 252 w)
+  const uint32_t *cdf;  // [LM_NCDF]
+  const int32_t *blk_carry;
+  uint32_t *psino, *dsino, *ssr, *phc, *dhc, *pvs, *fan;
+  unsigned long long *zR, *zM;
+  unsigned long long *sgl_idx;
+  int32_t *sgl_word, *sgl_ms;
+  unsigned int *sgl_count;
+  unsigned int sgl_cap;
+  unsigned int *bad;
+};
+
+__device__ inline bool is_tag(int32_t w) { return (w >> 29) == -4; }
+
+// last time tag (ms) of every block, -1 if none
+__global__ __launch_bounds__(LM_THREADS) void k_lm_tags(const int32_t *words, int64_t n,
+                                                         int32_t *blk_last) {
+  __shared__ long long best[LM_THREADS / 64];
+  const int64_t b0 = (int64_t)blockIdx.x * LM_BLOCK;
+  long long key = -1;  // (index << 32) | ms of the last tag seen by this lane
+  for (int j = 0; j < LM_PER_LANE; j++) {
+    int64_t i = b0 + (int64_t)j * LM_THREADS + threadIdx.x;
+    if (i < n) {
+      int32_t w = words[i];
+      if (is_tag(w)) key = ((long long)(i - b0) << 32) | (long long)(w & TMSK);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    long long v = __shfl_xor(key, o);
+    key = v > key ? v : key;
+  }
+  if ((threadIdx.x & 63) == 0) best[threadIdx.x >> 6] = key;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long k = -1;
+    for (int i = 0; i < LM_THREADS / 64; i++) k = best[i] > k ? best[i] : k;
+    blk_last[blockIdx.x] = k < 0 ? -1 : (int32_t)(k & 0xffffffffll);
+  }
+}
+
+// blk_carry[b] = last tag before block b (carry[0] from the previous chunk);
+// carry[0] <- last tag of the chunk. One block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_lm_carry(const int32_t *blk_last, int32_t *blk_carry,
+                                                   int nb, int32_t *carry) {
+  __shared__ int32_t part[1024];
+  const int t = threadIdx.x;
+  const int per = (nb + 1023) / 1024;
+  const int b0 = t * per, b1 = min(nb, b0 + per);
+  int32_t last = -1;
+  for (int b = b0; b < b1; b++)
+    if (blk_last[b] >= 0) last = blk_last[b];
+  part[t] = last;
+  __syncthreads();
+  // inclusive "last valid" scan over the 1024 partials (Hillis-Steele)
+  for (int o = 1; o < 1024; o <<= 1) {
+    int32_t v = t >= o ? part[t - o] : -1;
+    __syncthreads();
+    if (part[t] < 0 && v >= 0) part[t] = v;
+    __syncthreads();
+  }
+  int32_t c = t == 0 ? carry[0] : (part[t - 1] >= 0 ? part[t - 1] : carry[0]);
+  for (int b = b0; b < b1; b++) {
+    blk_carry[b] = c;
+    if (blk_last[b] >= 0) c = blk_last[b];
+  }
+  __syncthreads();
+  if (t == 1023 && part[1023] >= 0) carry[0] = part[1023];
+}
+
+__device__ inline uint32_t poisson_n(uint32_t u, const uint32_t *cdf) {
+  uint32_t N = 0;
+  while (N < LM_NCDF - 1 && u >= cdf[N]) N++;
+  return N;
+}
+
+// REP = false: one event per lane (nrep == 1).
+// REP = true: the wave walks its events one by one, lane r = replicate r.
+template <bool REP>
+__global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
+  __shared__ int32_t lut[NSN1];
+  __shared__ uint32_t cdf[LM_NCDF];
+  __shared__ uint32_t hcp[LM_HW * (REP ? LM_MAXREP : 1)];
+  __shared__ uint32_t hcd[LM_HW * (REP ? LM_MAXREP : 1)];
+  __shared__ unsigned long long mzr[LM_HW], mzm[LM_HW];
+  __shared__ int32_t wave_last[LM_WAVES];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nr = REP ? p.nrep : 1;
+  for (int i = tid; i < NSN1; i += LM_THREADS) lut[i] = p.lut[i];
+  if (tid < LM_NCDF) cdf[tid] = p.cdf[tid];
+  for (int i = tid; i < LM_HW * nr; i += LM_THREADS) {
+    hcp[i] = 0;
+    hcd[i] = 0;
+  }
+  if (tid < LM_HW) {
+    mzr[tid] = 0;
+    mzm[tid] = 0;
+  }
+  const int64_t b0 = (int64_t)blockIdx.x * LM_BLOCK;
+  const int64_t w0 = b0 + (int64_t)wave * (LM_PER_LANE * 64);
+  const int32_t carry_blk = p.blk_carry[blockIdx.x];
+  const int itag0 = carry_blk < 0 ? 0 : (carry_blk - p.toff) / ITIME;  // window base
+
+  int32_t wd[LM_PER_LANE];
+#pragma unroll
+  for (int j = 0; j < LM_PER_LANE; j++) {
+    int64_t i = w0 + j * 64 + lane;
+    wd[j] = i < p.n ? p.words[i] : 0;
+  }
+  // pass 1: last tag of each wave's words
+  int32_t wl = -1;
+#pragma unroll
+  for (int j = 0; j < LM_PER_LANE; j++) {
+    unsigned long long m = __ballot(is_tag(wd[j]));
+    if (m) wl = __builtin_amdgcn_readlane(wd[j] & TMSK, 63 - __clzll(m));
+  }
+  if (lane == 0) wave_last[wave] = wl;
+  __syncthreads();
+  int32_t c = carry_blk;
+  for (int k = 0; k < wave; k++)
+    if (wave_last[k] >= 0) c = wave_last[k];
+
+  const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1);
+  for (int j = 0; j < LM_PER_LANE; j++) {
+    const int32_t word = wd[j];
+    const int64_t gi = w0 + j * 64 + lane;
+    int32_t ms = c;
+    unsigned long long m = __ballot(is_tag(word));
+    if (m) {
+      unsigned long long ml = m & le;
+      int32_t tv = word & TMSK;
+      int src = ml ? 63 - __clzll(ml) : lane;
+      int32_t got = __shfl(tv, src);
+      if (ml) ms = got;
+      c = __builtin_amdgcn_readlane(tv, 63 - __clzll(m));
+    }
+    const int rel = ms < 0 ? 0 : ms - p.toff;
+    const int itag = rel / ITIME;
+    const bool in_frame = itag >= p.tstart && itag < p.tstop;
+    const int32_t val = word & 0x3fffffff;
+    bool ev = word > 0 && in_frame;
+    if (ev && val >= NSN1 * NSBINANG) {
+      atomicAdd(p.bad, 1u);
+      ev = false;
+    }
+    // singles reports: appended for the order-free bucket passes
+    if ((word >> 29) == -3 && in_frame) {
+      unsigned int k = atomicAdd(p.sgl_count, 1u);
+      if (k < p.sgl_cap) {
+        p.sgl_idx[k] = (unsigned long long)(p.base + gi);
+        p.sgl_word[k] = word;
+        p.sgl_ms[k] = rel;
+      }
+    }
+    if (!REP) {
+      if (!ev) continue;
+      uint32_t N = 1;
+      if (p.btp == 2) {
+        uint64_t g = (uint64_t)(p.base + gi);
+        uint32_t cc[4] = {(uint32_t)g, (uint32_t)(g >> 32), 0u, 0u};
+        philox4x32_10(cc, p.k0, p.k1);
+        N = poisson_n(cc[0], cdf);
+      }
+      if (N == 0 || N >= 32) continue;
+      const int si = val / NSBINANG, aw = val - si * NSBINANG;
+      const int a = aw / NSBINS, w = aw - a * NSBINS;
+      const int32_t lu = lut[si];
+      const int s11 = lu >> 16, sr = lu & 0xffff;
+      const int64_t addr = p.span == 11 ? (int64_t)s11 * NSBINANG + aw
+                                        : (p.span == 1 ? (int64_t)val : (int64_t)sr * NSBINANG + aw);
+      const int k = itag - itag0;
+      const bool inwin = k >= 0 && k < LM_HW;
+      if ((word >> 30) & 1) {
+        atomicAdd(p.psino + addr, N);
+        if (inwin) atomicAdd(&hcp[k], N); else atomicAdd(p.phc + itag, N);
+        atomicAdd(p.ssr + (int64_t)sr * NSBINANG + aw, N);
+        if (inwin) {
+          atomicAdd(&mzr[k], (unsigned long long)sr);
+          atomicAdd(&mzm[k], (unsigned long long)N);
+        } else {
+          atomicAdd(p.zR + itag, (unsigned long long)sr);
+          atomicAdd(p.zM + itag, (unsigned long long)N);
+        }
+        if ((a == 0 || a == 126) && itag < MXNITAG)
+          atomicAdd(p.pvs + (int64_t)(itag >> VTIME) * SEG0 * NSBINS + sr * NSBINS + w,
+                    N << (8 * (a == 126)));
+      } else {
+        atomicAdd(p.dsino + addr, N);
+        if (inwin) atomicAdd(&hcd[k], N); else atomicAdd(p.dhc + itag, N);
+        short2 rr = p.rno[si], cr = p.s2cF[a + NSANGLES * w];
+        atomicAdd(p.fan + NCRS * rr.x + cr.x, N);
+        atomicAdd(p.fan + NCRS * rr.y + cr.y, N);
+      }
+    } else {
+      unsigned long long evm = __ballot(ev);
+      while (evm) {
+        const int l = __ffsll((long long)evm) - 1;
+        evm &= evm - 1;
+        const int32_t wv = __builtin_amdgcn_readlane(word, l);
+        const int it = __builtin_amdgcn_readlane(itag, l);
+        const uint64_t g = (uint64_t)(p.base + w0 + j * 64 + l);
+        const int32_t v = wv & 0x3fffffff;
+        const int si = v / NSBINANG, aw = v - si * NSBINANG;
+        const int a = aw / NSBINS, w = aw - a * NSBINS;
+        const int32_t lu = lut[si];
+        const int s11 = lu >> 16, sr = lu & 0xffff;
+        const int64_t addr = p.span == 11 ? (int64_t)s11 * NSBINANG + aw
+                                          : (p.span == 1 ? (int64_t)v : (int64_t)sr * NSBINANG + aw);
+        uint32_t N = 0;
+        if (lane < nr) {
+          N = 1;
+          if (p.btp == 2) {
+            uint32_t cc[4] = {(uint32_t)g, (uint32_t)(g >> 32), (uint32_t)lane, 0u};
+            philox4x32_10(cc, p.k0, p.k1);
+            N = poisson_n(cc[0], cdf);
+          }
+        }
+        const bool ok = N > 0 && N < 32;
+        const int k = it - itag0;
+        const bool inwin = k >= 0 && k < LM_HW;
+        const bool prompt = (wv >> 30) & 1;
+        if (ok) {
+          atomicAdd((prompt ? p.psino : p.dsino) + addr * nr + lane, N);
+          uint32_t *hl = prompt ? hcp : hcd;
+          uint32_t *hg = prompt ? p.phc : p.dhc;
+          if (inwin) atomicAdd(&hl[k * nr + lane], N); else atomicAdd(hg + (int64_t)it * nr + lane, N);
+        }
+        if (lane == 0 && ok) {
+          if (prompt) {
+            atomicAdd(p.ssr + (int64_t)sr * NSBINANG + aw, N);
+            if (inwin) {
+              atomicAdd(&mzr[k], (unsigned long long)sr);
+              atomicAdd(&mzm[k], (unsigned long long)N);
+            } else {
+              atomicAdd(p.zR + it, (unsigned long long)sr);
+              atomicAdd(p.zM + it, (unsigned long long)N);
+            }
+            if ((a == 0 || a == 126) && it < MXNITAG)
+              atomicAdd(p.pvs + (int64_t)(it >> VTIME) * SEG0 * NSBINS + sr * NSBINS + w,
+                        N << (8 * (a == 126)));
+          } else {
+            short2 rr = p.rno[si], cr = p.s2cF[a + NSANGLES * w];
+            atomicAdd(p.fan + NCRS * rr.x + cr.x, N);
+            atomicAdd(p.fan + NCRS * rr.y + cr.y, N);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // flush the LDS windows
+  for (int i = tid; i < LM_HW * nr; i += LM_THREADS) {
+    const int it = itag0 + i / nr;
+    if (it >= p.nitag) continue;
+    if (hcp[i]) atomicAdd(p.phc + (int64_t)it * nr + i % nr, hcp[i]);
+    if (hcd[i]) atomicAdd(p.dhc + (int64_t)it * nr + i % nr, hcd[i]);
+  }
+  if (tid < LM_HW && itag0 + tid < p.nitag) {
+    if (mzr[tid]) atomicAdd(p.zR + itag0 + tid, mzr[tid]);
+    if (mzm[tid]) atomicAdd(p.zM + itag0 + tid, mzm[tid]);
+  }
+}
+
+// Singles buckets (lm/src/hst.cu:215-235). The reference adds itagu to the
+// second counter unshifted when that counter is still zero, shifted by 10
+// otherwise; a counter is zero before report k exactly when every earlier
+// report of the same (second, bucket) had value 0 and itagu 0. Pass 1 finds,
+// per (second, bucket), the first report with a non-zero value or itagu.
+__global__ void k_lm_sgl_first(const unsigned long long *idx, const int32_t *word,
+                               const int32_t *ms, const unsigned int *count, unsigned int cap,
+                               int nitag, unsigned long long *first_nz) {
+  unsigned int n = min(*count, cap);
+  for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    int32_t w = word[i];
+    unsigned int ib = (unsigned int)((w & TMSK) >> 19);
+    int itag = ms[i] / ITIME, itagu = ms[i] - itag * ITIME;
+    if (ib >= NBUCKTS || itag >= nitag) continue;
+    if ((w & 0x7ffff) != 0 || itagu != 0) atomicMin(first_nz + (int64_t)itag * NBUCKTS + ib, idx[i]);
+  }
+}
+
+__global__ void k_lm_sgl_apply(const unsigned long long *idx, const int32_t *word,
+                               const int32_t *ms, const unsigned int *count, unsigned int cap,
+                               int nitag, const unsigned long long *first_nz, uint32_t *bck) {
+  unsigned int n = min(*count, cap);
+  for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    int32_t w = word[i];
+    unsigned int ib = (unsigned int)((w & TMSK) >> 19);
+    int itag = ms[i] / ITIME, itagu = ms[i] - itag * ITIME;
+    if (ib >= NBUCKTS || itag >= nitag) continue;
+    uint32_t v = (uint32_t)(w & 0x7ffff);
+    int64_t k = (int64_t)itag * NBUCKTS + ib;
+    bool zero = v == 0 && idx[i] <= first_nz[k];
+    atomicAdd(bck + k, v << 3);
+    atomicAdd(bck + k + (int64_t)NBUCKTS * nitag,
+              ((uint32_t)(v > 0) << 30) + (zero ? (uint32_t)itagu : (uint32_t)itagu << 10));
+  }
+}
+
+__global__ void k_fill_u64(unsigned long long *p, unsigned long long v, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_philox_kat(const uint32_t *ctr, const uint32_t *key, uint32_t *out) {
+  uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+  philox4x32_10(c, key[0], key[1]);
+  for (int i = 0; i < 4; i++) out[i] = c[i];
+}
+
+}  // namespace nipet
+
+using namespace nipet;
+
+struct NipetLm {
+  NipetLmConfig cfg;
+  int device = 0;
+  int32_t *lut = nullptr;
+  short2 *rno = nullptr, *s2cF = nullptr;
+  uint32_t *cdf = nullptr;
+  int32_t *carry = nullptr;  // device: last tag (ms) seen so far
+  int32_t *blk_last = nullptr, *blk_carry = nullptr;
+  int64_t blk_cap = 0;
+  unsigned long long *first_nz = nullptr;
+  unsigned long long *sgl_idx = nullptr;
+  int32_t *sgl_word = nullptr, *sgl_ms = nullptr;
+  unsigned int *sgl_count = nullptr, *bad = nullptr;
+  unsigned int sgl_cap = 0;
+  int64_t base = 0;  // global index of the next word
+};
+
+static void lm_free(NipetLm *h) {
+  void *ps[] = {h->lut, h->rno, h->s2cF, h->cdf, h->carry, h->blk_last, h->blk_carry,
+                h->first_nz, h->sgl_idx, h->sgl_word, h->sgl_ms, h->sgl_count, h->bad};
+  for (void *q : ps) (void)hipFree(q);
+}
+
+// uint32 Poisson CDF (same formula and order as the CPU definition)
+static void poisson_cdf(double lambda, uint32_t *cdf, int K) {
+  double p = std::exp(-lambda), s = 0.0;
+  for (int k = 0; k < K; k++) {
+    s += p;
+    double v = std::floor(s * 4294967296.0);
+    cdf[k] = (k == K - 1 || v >= 4294967295.0) ? 0xFFFFFFFFu : (uint32_t)v;
+    p = p * lambda / (double)(k + 1);
+  }
+}
+
+static int lm_build(const NipetLmConfig *c, NipetLm *h) {
+  NIPET_HIP(hipGetDevice(&h->device));
+  std::vector<int32_t> lut(NSN1);
+  for (int i = 0; i < NSN1; i++) {
+    if (c->sn1_sn11[i] < 0 || c->sn1_ssrb[i] < 0 || c->sn1_ssrb[i] >= SEG0) {
+      set_error("nipet_lm_create: sn1_sn11/sn1_ssrb[%d] out of range", i);
+      return NIPET_EINVAL;
+    }
+    lut[i] = ((int32_t)c->sn1_sn11[i] << 16) | (int32_t)c->sn1_ssrb[i];
+  }
+  for (int i = 0; i < 2 * NSN1; i++)
+    if (c->sn1_rno[i] < 0 || c->sn1_rno[i] >= 64) {
+      set_error("nipet_lm_create: sn1_rno out of range");
+      return NIPET_EINVAL;
+    }
+  for (int i = 0; i < 2 * NSBINANG; i++)
+    if (c->s2cF[i] < 0 || c->s2cF[i] >= NCRS) {
+      set_error("nipet_lm_create: s2cF out of range");
+      return NIPET_EINVAL;
+    }
+  uint32_t cdf[LM_NCDF];
+  poisson_cdf(c->btprt, cdf, LM_NCDF);
+  NIPET_HIP(hipMalloc(&h->lut, NSN1 * sizeof(int32_t)));
+  NIPET_HIP(hipMalloc(&h->rno, NSN1 * sizeof(short2)));
+  NIPET_HIP(hipMalloc(&h->s2cF, NSBINANG * sizeof(short2)));
+  NIPET_HIP(hipMalloc(&h->cdf, LM_NCDF * sizeof(uint32_t)));
+  NIPET_HIP(hipMalloc(&h->carry, sizeof(int32_t)));
+  NIPET_HIP(hipMalloc(&h->sgl_count, sizeof(unsigned int)));
+  NIPET_HIP(hipMalloc(&h->bad, sizeof(unsigned int)));
+  NIPET_HIP(hipMalloc(&h->first_nz, (size_t)c->nitag * NBUCKTS * sizeof(unsigned long long)));
+  NIPET_HIP(hipMemcpy(h->lut, lut.data(), NSN1 * sizeof(int32_t), hipMemcpyHostToDevice));
+  NIPET_HIP(hipMemcpy(h->rno, c->sn1_rno, NSN1 * sizeof(short2), hipMemcpyHostToDevice));
+  NIPET_HIP(hipMemcpy(h->s2cF, c->s2cF, NSBINANG * sizeof(short2), hipMemcpyHostToDevice));
+  NIPET_HIP(hipMemcpy(h->cdf, cdf, sizeof(cdf), hipMemcpyHostToDevice));
+  int32_t none = -1;
+  NIPET_HIP(hipMemcpy(h->carry, &none, sizeof(int32_t), hipMemcpyHostToDevice));
+  NIPET_HIP(hipMemset(h->bad, 0, sizeof(unsigned int)));
+  int64_t nf = (int64_t)c->nitag * NBUCKTS;
+  k_fill_u64<<<(int)std::min<int64_t>((nf + 255) / 256, 4096), 256>>>(h->first_nz, ~0ull, nf);
+  NIPET_HIP(hipGetLastError());
+  NIPET_HIP(hipDeviceSynchronize());
+  return NIPET_OK;
+}
+
+extern "C" {
+
+int nipet_lm_create(const NipetLmConfig *c, NipetLm **out) {
+  if (!c || !out || !c->sn1_sn11 || !c->sn1_ssrb || !c->sn1_rno || !c->s2cF) {
+    set_error("nipet_lm_create: NULL argument");
+    return NIPET_EINVAL;
+  }
+  if (c->nitag <= 0 || c->tstart < 0 || c->tstop < c->tstart ||
+      (c->span != 11 && c->span != 1 && c->span != 0) || (c->btp != 0 && c->btp != 2) ||
+      c->nrep < 1 || c->nrep > LM_MAXREP || !(c->btprt > 0.0 && c->btprt < 20.0)) {
+    set_error("nipet_lm_create: bad configuration (nitag=%d span=%d btp=%d nrep=%d)", c->nitag,
+              c->span, c->btp, c->nrep);
+    return NIPET_EINVAL;
+  }
+  NipetLm *h = new NipetLm();
+  h->cfg = *c;
+  int rc = lm_build(c, h);
+  if (rc) {
+    lm_free(h);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return NIPET_OK;
+}
+
+void nipet_lm_destroy(NipetLm *h) {
+  if (!h) return;
+  lm_free(h);
+  delete h;
+}
+
+static int grow(NipetLm *h, int64_t nblk, unsigned int sgl) {
+  if (nblk > h->blk_cap) {
+    (void)hipFree(h->blk_last);
+    (void)hipFree(h->blk_carry);
+    NIPET_HIP(hipMalloc(&h->blk_last, nblk * sizeof(int32_t)));
+    NIPET_HIP(hipMalloc(&h->blk_carry, nblk * sizeof(int32_t)));
+    h->blk_cap = nblk;
+  }
+  if (sgl > h->sgl_cap) {
+    (void)hipFree(h->sgl_idx);
+    (void)hipFree(h->sgl_word);
+    (void)hipFree(h->sgl_ms);
+    NIPET_HIP(hipMalloc(&h->sgl_idx, (size_t)sgl * sizeof(unsigned long long)));
+    NIPET_HIP(hipMalloc(&h->sgl_word, (size_t)sgl * sizeof(int32_t)));
+    NIPET_HIP(hipMalloc(&h->sgl_ms, (size_t)sgl * sizeof(int32_t)));
+    h->sgl_cap = sgl;
+  }
+  return NIPET_OK;
+}
+
+int nipet_lm_chunk(NipetLm *h, const int32_t *words, int64_t n, const NipetLmOut *o,
+                   void *stream) {
+  if (!h || !o || (!words && n > 0) || n < 0) {
+    set_error("nipet_lm_chunk: bad argument");
+    return NIPET_EINVAL;
+  }
+  if (!o->psino || !o->dsino || !o->ssr || !o->phc || !o->dhc || !o->zR || !o->zM || !o->pvs ||
+      !o->fan || !o->bck) {
+    set_error("nipet_lm_chunk: every output buffer is required");
+    return NIPET_EINVAL;
+  }
+  if (n == 0) return NIPET_OK;
+  const NipetLmConfig &c = h->cfg;
+  hipStream_t st = (hipStream_t)stream;
+  int64_t nblk = (n + LM_BLOCK - 1) / LM_BLOCK;
+  if (nblk > INT32_MAX) {
+    set_error("nipet_lm_chunk: chunk too large");
+    return NIPET_EINVAL;
+  }
+  // singles list: at most one entry per word; size it generously for the
+  // chunk, entries beyond the capacity are reported as an error
+  unsigned int cap = (unsigned int)std::min<int64_t>(std::max<int64_t>(n / 32, 1 << 20), 1 << 28);
+  int rc = grow(h, nblk, cap);
+  if (rc) return rc;
+  NIPET_HIP(hipMemsetAsync(h->sgl_count, 0, sizeof(unsigned int), st));
+  prof_begin(PK_HIST, st);
+  k_lm_tags<<<(unsigned)nblk, LM_THREADS, 0, st>>>(words, n, h->blk_last);
+  k_lm_carry<<<1, 1024, 0, st>>>(h->blk_last, h->blk_carry, (int)nblk, h->carry);
+  LmDev p;
+  p.words = words;
+  p.n = n;
+  p.base = h->base;
+  p.toff = c.toff;
+  p.nitag = c.nitag;
+  p.tstart = c.tstart;
+  p.tstop = c.tstop;
+  p.span = c.span;
+  p.btp = c.btp;
+  p.nrep = c.nrep;
+  p.k0 = (uint32_t)c.seed;
+  p.k1 = (uint32_t)(c.seed >> 32);
+  p.nbins = 0;
+  p.lut = h->lut;
+  p.rno = h->rno;
+  p.s2cF = h->s2cF;
+  p.cdf = h->cdf;
+  p.blk_carry = h->blk_carry;
+  p.psino = o->psino;
+  p.dsino = o->dsino;
+  p.ssr = o->ssr;
+  p.phc = o->phc;
+  p.dhc = o->dhc;
+  p.pvs = o->pvs;
+  p.fan = o->fan;
+  p.zR = (unsigned long long *)o->zR;
+  p.zM = (unsigned long long *)o->zM;
+  p.sgl_idx = h->sgl_idx;
+  p.sgl_word = h->sgl_word;
+  p.sgl_ms = h->sgl_ms;
+  p.sgl_count = h->sgl_count;
+  p.sgl_cap = h->sgl_cap;
+  p.bad = h->bad;
+  if (c.nrep == 1)
+    k_lm_hist<false><<<(unsigned)nblk, LM_THREADS, 0, st>>>(p);
+  else
+    k_lm_hist<true><<<(unsigned)nblk, LM_THREADS, 0, st>>>(p);
+  k_lm_sgl_first<<<256, 256, 0, st>>>(h->sgl_idx, h->sgl_word, h->sgl_ms, h->sgl_count,
+                                      h->sgl_cap, c.nitag, h->first_nz);
+  k_lm_sgl_apply<<<256, 256, 0, st>>>(h->sgl_idx, h->sgl_word, h->sgl_ms, h->sgl_count,
+                                      h->sgl_cap, c.nitag, h->first_nz, o->bck);
+  prof_end(PK_HIST, st);
+  NIPET_HIP(hipGetLastError());
+  h->base += n;
+  return NIPET_OK;
+}
+
+int nipet_lm_status(NipetLm *h, int64_t *words_done, unsigned int *bad_words,
+                    unsigned int *singles_overflow) {
+  if (!h) {
+    set_error("nipet_lm_status: NULL handle");
+    return NIPET_EINVAL;
+  }
+  unsigned int bad = 0, cnt = 0;
+  NIPET_HIP(hipMemcpy(&bad, h->bad, sizeof(bad), hipMemcpyDeviceToHost));
+  NIPET_HIP(hipMemcpy(&cnt, h->sgl_count, sizeof(cnt), hipMemcpyDeviceToHost));
+  if (words_done) *words_done = h->base;
+  if (bad_words) *bad_words = bad;
+  if (singles_overflow) *singles_overflow = cnt > h->sgl_cap ? cnt - h->sgl_cap : 0;
+  return NIPET_OK;
+}
+
+int nipet_philox4x32(const uint32_t *ctr, const uint32_t *key, uint32_t *out) {
+  if (!ctr || !key || !out) {
+    set_error("nipet_philox4x32: NULL argument");
+    return NIPET_EINVAL;
+  }
+  uint32_t *d = nullptr;
+  NIPET_HIP(hipMalloc(&d, 10 * sizeof(uint32_t)));
+  NIPET_HIP(hipMemcpy(d, ctr, 4 * sizeof(uint32_t), hipMemcpyHostToDevice));
+  NIPET_HIP(hipMemcpy(d + 4, key, 2 * sizeof(uint32_t), hipMemcpyHostToDevice));
+  k_philox_kat<<<1, 1>>>(d, d + 4, d + 6);
+  NIPET_HIP(hipGetLastError());
+  NIPET_HIP(hipMemcpy(out, d + 6, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  (void)hipFree(d);
+  return NIPET_OK;
+}
+
+}  // extern "C"

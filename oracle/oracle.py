@@ -36,6 +36,12 @@ def lib():
         for f in (L.orc_tx_lists, L.orc_fprj, L.orc_bprj):
             f.restype = ct.c_int
         L.orc_max_threads.restype = ct.c_int
+        L.orc_philox4x32.argtypes = [P, P, P]
+        L.orc_poisson_cdf.argtypes = [ct.c_double, P, ct.c_int]
+        L.orc_lm_hist.argtypes = [P, ct.c_int64, ct.c_int64, ct.c_int, ct.c_int, ct.c_int,
+                                  ct.c_int, ct.c_int, P, P, P, P, ct.c_int, P, ct.c_int,
+                                  ct.c_uint64, ct.c_int, ct.c_int64] + [P] * 10
+        L.orc_lm_hist.restype = ct.c_int64
         _LIB = L
     return _LIB
 
@@ -128,3 +134,47 @@ def osem(params, prompts, rho, sens, mask, subsets, img0, niter, eps=1e-6):
             b = np.where(np.abs(sv) < eps, 0.0, b / np.where(np.abs(sv) < eps, 1.0, sv))
             x = np.where(mask, x * b, 0.0)
     return x
+
+
+def philox4x32(ctr, key):
+    """Philox4x32-10 of one 4-word counter and 2-word key."""
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32(_p(c), _p(k), _p(out))
+    return out
+
+
+def poisson_cdf(lam, K=40):
+    cdf = np.zeros(K, np.uint32)
+    lib().orc_poisson_cdf(float(lam), _p(cdf), K)
+    return cdf
+
+
+def lm_hist(words, params, toff, nitag, tstart=0, tstop=None, span=11, btp=0, btprt=1.0,
+            seed=0, nrep=1, base=0):
+    """Sequential histogram of an mMR list-mode word stream (see lm_oracle.c).
+    Returns a dict of uint32 / uint64 arrays (prompt and delayed counts separate)."""
+    Cnt, tx, ax = params['Cnt'], params['txLUT'], params['axLUT']
+    words = np.ascontiguousarray(words, dtype=np.int32)
+    tstop = nitag if tstop is None else tstop
+    nbins = {11: Cnt['NSN11'], 1: Cnt['NSN1'], 0: Cnt['NSEG0']}[span] * 86688
+    tn = (min(nitag, 5400) + 3) // 4
+    out = dict(psino=np.zeros((nrep, nbins), np.uint32), dsino=np.zeros((nrep, nbins), np.uint32),
+               ssr=np.zeros(127 * 86688, np.uint32), phc=np.zeros((nrep, nitag), np.uint32),
+               dhc=np.zeros((nrep, nitag), np.uint32), zR=np.zeros(nitag, np.uint64),
+               zM=np.zeros(nitag, np.uint64), pvs=np.zeros(tn * 127 * 344, np.uint32),
+               fan=np.zeros(64 * 504, np.uint32), bck=np.zeros(2 * nitag * 224, np.uint32))
+    cdf = poisson_cdf(btprt)
+    sn1_sn11 = np.ascontiguousarray(ax['sn1_sn11'], np.int16)
+    sn1_ssrb = np.ascontiguousarray(ax['sn1_ssrb'], np.int16)
+    sn1_rno = np.ascontiguousarray(ax['sn1_rno'], np.int16)
+    s2cF = np.ascontiguousarray(tx['s2cF'], np.int16)
+    n = lib().orc_lm_hist(_p(words), len(words), base, toff, nitag, tstart, tstop, span,
+                          _p(sn1_sn11), _p(sn1_ssrb), _p(sn1_rno), _p(s2cF), btp, _p(cdf),
+                          len(cdf), seed, nrep, nbins,
+                          *[_p(out[k]) for k in ('psino', 'dsino', 'ssr', 'phc', 'dhc', 'zR', 'zM',
+                                                 'pvs', 'fan', 'bck')])
+    assert n >= 0, 'bad list-mode address'
+    out['counted'] = n
+    return out

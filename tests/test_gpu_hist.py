@@ -1,0 +1,126 @@
+"""GPU list-mode histogramming, bit-exact against the sequential CPU oracle
+(oracle/lm_oracle.c) on every output: span-11 prompt/delayed sinograms,
+SSRB sinogram, head curves, centre-of-mass sums, projection views, fan-sums
+and singles buckets; plain, bootstrap, frame window, chunked and replicates."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+OUT = ('psino', 'dsino', 'ssr', 'phc', 'dhc', 'zR', 'zM', 'pvs', 'fan', 'bck')
+
+
+@pytest.fixture(scope='module')
+def torch():
+    import torch
+    assert torch.cuda.is_available(), 'GPU tests need a HIP device'
+    return torch
+
+
+@pytest.fixture(scope='module')
+def stream(torch, params):
+    """6 s synthetic stream (1 ms tags, singles each second) with events
+    before the first tag, zero words and out-of-range addresses mixed in."""
+    from nipet_amd.lm.lmsim import synth_lm
+    w = synth_lm(params, 6, 400, seed=7).cpu().numpy()
+    rng = np.random.default_rng(1)
+    pre = ((1 << 30) | rng.integers(86688, 4084 * 86688, 300)).astype(np.int32)
+    junk_at = rng.choice(np.arange(10, len(w) - 10), 50, replace=False)
+    w[junk_at[:25]] = 0
+    w[junk_at[25:]] = (1 << 30) | 0x3ffffff0
+    return np.concatenate([pre, w])
+
+
+def _toff_nitag(w):
+    t = np.nonzero((w >> 29) == -4)[0]
+    tf, tl = int(w[t[0]] & 0x1fffffff), int(w[t[-1]] & 0x1fffffff)
+    return tf, ((tl - tf) + 999) // 1000
+
+
+def _gpu(torch, params, w, toff, nitag, chunks=(len,), **kw):
+    from nipet_amd.lm.mmrhist import LmHistogram
+    h = LmHistogram(params, toff, nitag, **kw)
+    wd = torch.from_numpy(w).cuda()
+    cuts = [0] + [c(w) if callable(c) else c for c in chunks]
+    for a, b in zip(cuts[:-1], cuts[1:]):
+        h.add(wd[a:b])
+    torch.cuda.synchronize()
+    return h, {k: v.cpu().numpy() for k, v in h.out.items()}
+
+
+def _compare(g, o, nrep):
+    for k in ('psino', 'dsino', 'phc', 'dhc'):
+        gv = g[k].astype(np.uint32)                       # [n][nrep]
+        ov = o[k].T                                       # [nrep][n] -> [n][nrep]
+        assert np.array_equal(gv, ov), k
+    for k in ('ssr', 'pvs', 'fan', 'bck'):
+        assert np.array_equal(g[k].astype(np.uint32).ravel(), o[k].ravel()), k
+    for k in ('zR', 'zM'):
+        assert np.array_equal(g[k].astype(np.uint64), o[k]), k
+
+
+def test_philox_kat_gpu(torch):
+    import ctypes as ct
+
+    from nipet_amd import _lib
+    kat = [([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+           ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+            [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1])]
+    for ctr, key, exp in kat:
+        c, k, o = (np.array(ctr, np.uint32), np.array(key, np.uint32), np.zeros(4, np.uint32))
+        _lib.check(_lib.lib().nipet_philox4x32(c.ctypes.data, k.ctypes.data, o.ctypes.data))
+        assert list(o) == exp
+
+
+@pytest.mark.parametrize('btp', [0, 2])
+def test_hist_bit_exact(torch, params, stream, btp):
+    from oracle import oracle as O
+    toff, nitag = _toff_nitag(stream)
+    h, g = _gpu(torch, params, stream, toff, nitag, btp=btp, seed=99)
+    o = O.lm_hist(stream, params, toff, nitag, btp=btp, seed=99)
+    _compare(g, o, 1)
+    n, bad, ovf = h.status()
+    assert n == len(stream) and bad == 25 and ovf == 0
+    # reference invariants (tests/test_amyloid_pvc.py:131-133)
+    assert g['phc'].sum() == g['psino'].sum() and g['dhc'].sum() == g['dsino'].sum()
+
+
+def test_hist_chunked_frame_window(torch, params, stream):
+    from oracle import oracle as O
+    toff, nitag = _toff_nitag(stream)
+    cuts = (1234, 700_001, 700_002, 1_500_000, len)
+    h, g = _gpu(torch, params, stream, toff, nitag, chunks=cuts, tstart=2, tstop=4, btp=2,
+                seed=5)
+    o = O.lm_hist(stream, params, toff, nitag, tstart=2, tstop=4, btp=2, seed=5)
+    _compare(g, o, 1)
+    assert g['phc'][:2].sum() == 0 and g['phc'][4:].sum() == 0
+
+
+def test_hist_replicates(torch, params, stream):
+    from oracle import oracle as O
+    toff, nitag = _toff_nitag(stream)
+    w = stream[:900_000]
+    h, g = _gpu(torch, params, w, toff, nitag, btp=2, seed=2024, nrep=3)
+    o = O.lm_hist(w, params, toff, nitag, btp=2, seed=2024, nrep=3)
+    _compare(g, o, 3)
+    # replicates differ from each other
+    assert not np.array_equal(g['psino'][:, 0], g['psino'][:, 1])
+
+
+def test_mmrhist_file(torch, params, stream, tmp_path):
+    import nipet_amd as nip
+    from oracle import oracle as O
+    f = os.path.join(tmp_path, 'lm.bf')
+    stream.tofile(f)
+    toff, nitag = _toff_nitag(stream)
+    ele, tt, _ = nip.lminfo(f)
+    assert ele == len(stream) and tt[0] == toff
+    hst = nip.mmrhist({'lm_bf': f}, params, t0=1, t1=5)
+    o = O.lm_hist(stream, params, toff, nitag, tstart=1, tstop=5)
+    assert hst['psino'].shape == (837, 252, 344) and hst['psino'].dtype == np.uint16
+    assert np.array_equal(hst['psino'].ravel(), (o['psino'][0] & 0xffff).astype(np.uint16))
+    assert np.array_equal(hst['dsino'].ravel(), (o['dsino'][0] & 0xffff).astype(np.uint16))
+    assert np.array_equal(hst['phc'], o['phc'][0]) and hst['dur'] == 4
+    assert hst['fansums'].shape == (64, 504) and hst['pssr'].shape == (127, 252, 344)

@@ -1,0 +1,93 @@
+"""Pinning the list-mode oracle (oracle/lm_oracle.c): Philox4x32-10 known-answer
+vectors of the published generator (Random123 kat_vectors), the Poisson CDF
+table, and a hand-built word stream whose every output is known."""
+import numpy as np
+
+from oracle import oracle as O
+
+TAG = -(1 << 31)
+
+
+def test_philox_known_answers():
+    kat = [([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+           ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+           ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+            [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1])]
+    for ctr, key, exp in kat:
+        assert list(O.philox4x32(ctr, key)) == exp
+
+
+def test_poisson_cdf_table():
+    from math import exp, factorial
+    c = O.poisson_cdf(1.0, 40).astype(np.float64)
+    p = np.cumsum([exp(-1) / factorial(k) for k in range(40)])
+    assert np.all(np.abs(c[:-1] - np.floor(p[:-1] * 2**32)) <= 1)
+    assert np.all(np.diff(c) >= 0) and c[-1] == 2**32 - 1
+
+
+def _addr(si, a, w):
+    return si * 86688 + a * 344 + w
+
+
+def test_hand_built_stream(params):
+    ax, tx = params['axLUT'], params['txLUT']
+    toff = 5000
+    P = 1 << 30
+    sgl = lambda b, v: -(1 << 31) + (1 << 29) | (b << 19) | v  # noqa: E731
+    words = np.array([
+        P | _addr(7, 5, 9),                 # before the first tag -> second 0
+        TAG | toff,                         # t = 0 s
+        P | _addr(100, 0, 17),              # a = 0 -> projection view
+        _addr(200, 10, 20),                 # delayed
+        sgl(3, 0),                          # empty report, itagu 0: counter stays 0
+        sgl(3, 0),
+        TAG | (toff + 1250),                # t = 1.25 s
+        P | _addr(100, 126, 17),            # a = 126 -> projection view << 8
+        sgl(3, 5),                          # first non-empty report
+        sgl(3, 0),                          # counter already non-zero -> itagu << 10
+        sgl(250, 1),                        # bucket >= 224 ignored
+        0,                                  # zero word: ignored
+        P | 0x3fffffff,                     # address beyond span-1: skipped
+        TAG | (toff + 3100),                # t = 3.1 s (outside the frame [0, 2))
+        P | _addr(1, 1, 1),
+        TAG | (toff + 3999),
+    ], dtype=np.int64).astype(np.int32)
+    nitag = 4
+    h = O.lm_hist(words, params, toff, nitag, tstart=0, tstop=2)
+    s11, ssrb, rno = (ax[k].astype(np.int64) for k in ('sn1_sn11', 'sn1_ssrb', 'sn1_rno'))
+    ps = np.zeros(837 * 86688, np.uint32)
+    for si, a, w in [(7, 5, 9), (100, 0, 17), (100, 126, 17)]:
+        ps[s11[si] * 86688 + a * 344 + w] += 1
+    assert np.array_equal(h['psino'][0], ps)
+    ds = np.zeros_like(ps)
+    ds[s11[200] * 86688 + 10 * 344 + 20] = 1
+    assert np.array_equal(h['dsino'][0], ds)
+    assert list(h['phc'][0]) == [2, 1, 0, 0] and list(h['dhc'][0]) == [1, 0, 0, 0]
+    assert h['counted'] == 4
+    assert list(h['zR']) == [ssrb[7] + ssrb[100], ssrb[100], 0, 0]
+    assert list(h['zM']) == [2, 1, 0, 0]
+    pv = h['pvs'].reshape(-1, 127, 344)
+    assert pv[0, ssrb[100], 17] == 1 + (1 << 8) and pv.sum() == 1 + (1 << 8)
+    c = tx['s2cF'][10 + 252 * 20].astype(np.int64)
+    fan = h['fan'].reshape(64, 504)
+    assert fan[rno[200, 0], c[0]] == 1 and fan[rno[200, 1], c[1]] == 1 and fan.sum() == 2
+    ssr = h['ssr'].reshape(127, 86688)
+    assert ssr[ssrb[100], 17] == 1 and ssr[ssrb[100], 126 * 344 + 17] == 1 and ssr.sum() == 3
+    bck = h['bck'].reshape(2, nitag, 224)
+    assert bck[0, 0, 3] == 0 and bck[1, 0, 3] == 0
+    assert bck[0, 1, 3] == 5 << 3
+    assert bck[1, 1, 3] == (1 << 30) + (250 << 10) + (250 << 10)
+    assert bck.sum() == bck[0, 1, 3] + bck[1, 1, 3]
+
+
+def test_bootstrap_weights_statistics(params):
+    # Poisson(1) weights: mean 1 and variance 1 over many events
+    n = 200_000
+    words = np.full(n + 2, (1 << 30) | _addr(10, 3, 3), np.int32)
+    words[0] = TAG | 0
+    words[-1] = TAG | 999
+    h = O.lm_hist(words, params, 0, 1, btp=2, btprt=1.0, seed=12345)
+    tot = int(h['phc'][0, 0])
+    assert abs(tot / n - 1.0) < 0.01
+    z0 = n - h['counted']                   # N == 0 draws
+    assert abs(z0 / n - np.exp(-1)) < 0.005
