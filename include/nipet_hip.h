@@ -144,7 +144,7 @@ typedef struct {
 /* Device outputs, zeroed by the caller, accumulated by every chunk. */
 typedef struct {
   uint32_t *psino, *dsino;  /* prompts / delayeds [nbins][nrep] (nbins = nsinos*86688) */
-  uint32_t *ssr;            /* SSRB prompts [127][252][344] (replicate 0) */
+  uint32_t *ssr;            /* SSRB prompts [127][252][344] (replicate 0), by nipet_lm_finish */
   uint32_t *phc, *dhc;      /* head curves [nitag][nrep] */
   uint64_t *zR, *zM;        /* centre-of-mass sums [nitag] (replicate 0) */
   uint32_t *pvs;            /* projection views [(min(nitag,5400)+3)/4][127][344] */
@@ -156,6 +156,9 @@ int nipet_lm_create(const NipetLmConfig *cfg, NipetLm **out);
 /* Histogram words[0..n) (device int32) on `stream`. */
 int nipet_lm_chunk(NipetLm *h, const int32_t *words, int64_t n, const NipetLmOut *out,
                    void *stream);
+/* After the last chunk: the SSRB prompt sinogram out->ssr (overwritten) from
+ * the replicate-0 prompt sinogram (each output sinogram lies on one SSRB plane). */
+int nipet_lm_finish(NipetLm *h, const NipetLmOut *out, void *stream);
 /* Words processed, words with an address beyond the span-1 range (skipped),
  * singles reports lost to list overflow. Synchronous. */
 int nipet_lm_status(NipetLm *h, int64_t *words_done, unsigned int *bad_words,

@@ -68,6 +68,7 @@ def lib():
             'nipet_lm_create': ([ct.POINTER(NipetLmConfig), ct.POINTER(P)], I),
             'nipet_lm_chunk': ([P, P, I64, ct.POINTER(NipetLmOut), P], I),
             'nipet_lm_status': ([P, P, P, P], I),
+            'nipet_lm_finish': ([P, ct.POINTER(NipetLmOut), P], I),
             'nipet_lm_destroy': ([P], None),
             'nipet_philox4x32': ([P, P, P], I),
             'nipet_prof_enable': ([I], I),

@@ -237,7 +237,6 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
       if ((word >> 30) & 1) {
         atomicAdd(p.psino + addr, N);
         if (inwin) atomicAdd(&hcp[k], N); else atomicAdd(p.phc + itag, N);
-        atomicAdd(p.ssr + (int64_t)sr * NSBINANG + aw, N);
         if (inwin) {
           atomicAdd(&mzr[k], (unsigned long long)sr);
           atomicAdd(&mzm[k], (unsigned long long)N);
@@ -291,7 +290,6 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
         }
         if (lane == 0 && ok) {
           if (prompt) {
-            atomicAdd(p.ssr + (int64_t)sr * NSBINANG + aw, N);
             if (inwin) {
               atomicAdd(&mzr[k], (unsigned long long)sr);
               atomicAdd(&mzm[k], (unsigned long long)N);
@@ -361,6 +359,22 @@ __global__ void k_lm_sgl_apply(const unsigned long long *idx, const int32_t *wor
   }
 }
 
+// SSRB prompt sinogram from the replicate-0 prompt sinogram: every span-11
+// (or span-1) sinogram s holds LORs of a single axial position ssrb(s), so
+// ssr[p][aw] = sum over s with ssrb(s) = p of psino[s][aw] (exact in integers).
+__global__ void k_lm_ssrb(const uint32_t *psino, int nrep, const int32_t *s_off,
+                          const int32_t *s_list, uint32_t *ssr) {
+  const int64_t n = (int64_t)SEG0 * NSBINANG;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int pl = (int)(i / NSBINANG), aw = (int)(i - (int64_t)pl * NSBINANG);
+    uint32_t acc = 0;
+    for (int k = s_off[pl]; k < s_off[pl + 1]; k++)
+      acc += psino[((int64_t)s_list[k] * NSBINANG + aw) * nrep];
+    ssr[i] = acc;
+  }
+}
+
 __global__ void k_fill_u64(unsigned long long *p, unsigned long long v, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -391,12 +405,14 @@ struct NipetLm {
   int32_t *sgl_word = nullptr, *sgl_ms = nullptr;
   unsigned int *sgl_count = nullptr, *bad = nullptr;
   unsigned int sgl_cap = 0;
+  int32_t *ssrb_off = nullptr, *ssrb_list = nullptr;  // sinograms of each SSRB plane
   int64_t base = 0;  // global index of the next word
 };
 
 static void lm_free(NipetLm *h) {
   void *ps[] = {h->lut, h->rno, h->s2cF, h->cdf, h->carry, h->blk_last, h->blk_carry,
-                h->first_nz, h->sgl_idx, h->sgl_word, h->sgl_ms, h->sgl_count, h->bad};
+                h->first_nz, h->sgl_idx, h->sgl_word, h->sgl_ms, h->sgl_count, h->bad,
+                h->ssrb_off, h->ssrb_list};
   for (void *q : ps) (void)hipFree(q);
 }
 
@@ -433,6 +449,34 @@ static int lm_build(const NipetLmConfig *c, NipetLm *h) {
     }
   uint32_t cdf[LM_NCDF];
   poisson_cdf(c->btprt, cdf, LM_NCDF);
+  // sinograms of the output span that fall on each SSRB plane
+  const int nsino = c->span == 11 ? 837 : (c->span == 1 ? NSN1 : SEG0);
+  std::vector<int> plane_of(nsino, -1);
+  for (int si = 0; si < NSN1; si++) {
+    int s = c->span == 11 ? c->sn1_sn11[si] : (c->span == 1 ? si : c->sn1_ssrb[si]);
+    if (s < 0 || s >= nsino) {
+      set_error("nipet_lm_create: span map out of range at %d", si);
+      return NIPET_EINVAL;
+    }
+    if (plane_of[s] >= 0 && plane_of[s] != c->sn1_ssrb[si]) {
+      set_error("nipet_lm_create: output sinogram %d mixes axial positions", s);
+      return NIPET_EINVAL;
+    }
+    plane_of[s] = c->sn1_ssrb[si];
+  }
+  std::vector<int32_t> soff(SEG0 + 1, 0), slist;
+  for (int pl = 0; pl < SEG0; pl++) {
+    for (int s = 0; s < nsino; s++)
+      if (plane_of[s] == pl) slist.push_back(s);
+    soff[pl + 1] = (int32_t)slist.size();
+  }
+  NIPET_HIP(hipMalloc(&h->ssrb_off, (SEG0 + 1) * sizeof(int32_t)));
+  NIPET_HIP(hipMalloc(&h->ssrb_list, std::max<size_t>(slist.size(), 1) * sizeof(int32_t)));
+  NIPET_HIP(hipMemcpy(h->ssrb_off, soff.data(), (SEG0 + 1) * sizeof(int32_t),
+                      hipMemcpyHostToDevice));
+  if (!slist.empty())
+    NIPET_HIP(hipMemcpy(h->ssrb_list, slist.data(), slist.size() * sizeof(int32_t),
+                        hipMemcpyHostToDevice));
   NIPET_HIP(hipMalloc(&h->lut, NSN1 * sizeof(int32_t)));
   NIPET_HIP(hipMalloc(&h->rno, NSN1 * sizeof(short2)));
   NIPET_HIP(hipMalloc(&h->s2cF, NSBINANG * sizeof(short2)));
@@ -580,6 +624,17 @@ int nipet_lm_chunk(NipetLm *h, const int32_t *words, int64_t n, const NipetLmOut
   prof_end(PK_HIST, st);
   NIPET_HIP(hipGetLastError());
   h->base += n;
+  return NIPET_OK;
+}
+
+int nipet_lm_finish(NipetLm *h, const NipetLmOut *o, void *stream) {
+  if (!h || !o || !o->psino || !o->ssr) {
+    set_error("nipet_lm_finish: bad argument");
+    return NIPET_EINVAL;
+  }
+  k_lm_ssrb<<<4096, 256, 0, (hipStream_t)stream>>>(o->psino, h->cfg.nrep, h->ssrb_off,
+                                                     h->ssrb_list, o->ssr);
+  NIPET_HIP(hipGetLastError());
   return NIPET_OK;
 }
 

@@ -119,6 +119,11 @@ class LmHistogram:
                                              ct.byref(self._o), _lib.stream_handle()),
                    'nipet_lm_chunk')
 
+    def finish(self):
+        """Derive the SSRB prompt sinogram; call after the last chunk."""
+        _lib.check(_lib.lib().nipet_lm_finish(self.handle, ct.byref(self._o),
+                                              _lib.stream_handle()), 'nipet_lm_finish')
+
     def status(self):
         n, bad, ovf = ct.c_int64(), ct.c_uint(), ct.c_uint()
         _lib.check(_lib.lib().nipet_lm_status(self.handle, ct.byref(n), ct.byref(bad),
@@ -139,6 +144,7 @@ def hist_words(words, scanner_params, t0=0, t1=0, btp=None, seed=0, nrep=1, span
     h = LmHistogram(scanner_params, tf, nitag, t0, min(t1, nitag), span=span, btp=btp, seed=seed,
                     nrep=nrep)
     h.add(w.to('cuda', torch.int32))
+    h.finish()
     return h
 
 
@@ -165,6 +171,7 @@ def _stream_file(h, flm, offset_words=0):
         h.add(dev[b][:e - s])
         done[b].record(main)
         k += 1
+    h.finish()
     torch.cuda.synchronize()
 
 

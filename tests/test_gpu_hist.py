@@ -46,6 +46,7 @@ def _gpu(torch, params, w, toff, nitag, chunks=(len,), **kw):
     cuts = [0] + [c(w) if callable(c) else c for c in chunks]
     for a, b in zip(cuts[:-1], cuts[1:]):
         h.add(wd[a:b])
+    h.finish()
     torch.cuda.synchronize()
     return h, {k: v.cpu().numpy() for k, v in h.out.items()}
 
