@@ -102,6 +102,16 @@ int nipet_remgaps(float *out, const void *in, int in_dtype, const int32_t *aw2al
 int nipet_putgaps(float *out, const float *in, const int32_t *aw2ali, int naw, int nsn,
                   int nbinang, void *stream);
 
+/* ---- batches: nb replicates (bootstrap replicates, frames) projected together,
+ * sharing the geometry traversal. Layouts are replicate-innermost:
+ * images [320][320][127][nb], sinograms [nprj][nsn][nb]. nb <= NIPET_MAXBATCH;
+ * span 11 only (nsn <= 837). Same semantics as the nb = 1 entry points. */
+#define NIPET_MAXBATCH 64
+int nipet_fprj_b(const NipetGeom *g, float *sino, const float *img, const int32_t *subs,
+                 int nprj, int nb, int att, void *stream);
+int nipet_bprj_b(const NipetGeom *g, float *img, const float *sino, const int32_t *subs,
+                 int nprj, int nb, void *stream);
+
 /* OSEM corrections, computed once per reconstruction (prj/src/recon.cu:381-398):
  *   rho[i] = (r[i] + s[i]) / (a[i] n[i])  (0 where |a n| < 1e-6),  an[i] = a[i] n[i].
  * r or s may be NULL (zero). All device arrays of n elements. */
@@ -117,6 +127,15 @@ int nipet_osem_corr(float *rho, float *an, const float *a, const float *n, const
 int nipet_osem_subset(const NipetGeom *g, float *img, const uint16_t *prompts, const float *rho,
                       const float *sens, const uint8_t *mask, const int32_t *subs, int nprj,
                       float *work_sino, float *work_img, void *stream);
+
+/* Batched OSEM subset update of nb images [voxel][nb]: prompts [naw][nsn][nb];
+ * rho shared [naw][nsn] (rho_per_rep = 0) or [naw][nsn][nb]; sens shared
+ * [voxel] (sens_per_rep = 0) or [voxel][nb]; mask shared; work_sino
+ * [nprj][nsn][nb]; work_img [voxel][nb]. */
+int nipet_osem_subset_b(const NipetGeom *g, float *img, const uint16_t *prompts,
+                        const float *rho, int rho_per_rep, const float *sens, int sens_per_rep,
+                        const uint8_t *mask, const int32_t *subs, int nprj, int nb,
+                        float *work_sino, float *work_img, void *stream);
 
 /* ---- list-mode histogramming (mmr_lmproc.hist: lm/src/lm_module.cu:174-391,
  * lmproc lm/src/lmproc.cu:12-230, gpu_hst / hst lm/src/hst.cu:37-528) ----

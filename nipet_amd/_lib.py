@@ -65,6 +65,9 @@ def lib():
             'nipet_remgaps': ([P, P, I, P, I, I, I, P], I),
             'nipet_putgaps': ([P, P, P, I, I, I, P], I),
             'nipet_osem_subset': ([P, P, P, P, P, P, P, I, P, P, P], I),
+            'nipet_fprj_b': ([P, P, P, P, I, I, I, P], I),
+            'nipet_bprj_b': ([P, P, P, P, I, I, P], I),
+            'nipet_osem_subset_b': ([P, P, P, P, I, P, I, P, P, I, I, P, P, P], I),
             'nipet_lm_create': ([ct.POINTER(NipetLmConfig), ct.POINTER(P)], I),
             'nipet_lm_chunk': ([P, P, I64, ct.POINTER(NipetLmOut), P], I),
             'nipet_lm_status': ([P, P, P, P], I),

@@ -22,14 +22,26 @@
 
 namespace nipet {
 
-constexpr int PRJ_THREADS = 704;  // 11 waves of 64
+constexpr int PRJ_THREADS = 704;  // FP: 11 waves of 64
 constexpr int LI_ROUNDS = 3;      // 3*704 = 2112 >= 2074 Michelogram elements
-constexpr int CHUNK = 32;         // transaxial segments per LDS stage
-constexpr int SUBSUM = 8;         // FP: float partial sums over 8 segments, folded in double
-constexpr int ZSTRIDE = 128;      // LDS floats per staged z-column (127 + zero pad)
-constexpr int ZGUARD = 8;         // zero guard floats on both sides of the stage
+constexpr int BP_THREADS = 512;   // BP: 8 waves
+constexpr int SUBSUM = 8;         // FP: float partial sums over 8 segments
+constexpr int ZSTRIDE = 128;      // LDS z-column (127 + zero pad)
+constexpr int ZGUARD = 8;         // zero guard z-slots on both sides of the stage
 constexpr double TFAR = 1e300;    // boundary "never" (direct planes)
 constexpr double SPAD = -1e30;    // start of a padding segment (contributes nothing)
+constexpr int MAXNB = 4;          // replicates per kernel instance (larger batches: sub-batches)
+constexpr int BP_MAXD = 63;       // largest ring difference of the BP tables
+
+template <int NB>
+constexpr int fp_chunk() { return NB >= 4 ? 16 : 32; }
+template <int NB>
+constexpr int bp_chunk() { return NB == 1 ? 32 : (NB == 2 ? 16 : 8); }
+template <int NB>
+constexpr int fp_bins() { return NB == 1 ? MAXNSN : 837 * NB; }
+// BP table row d holds r0 = -2 .. 65-d (two zero pads on both sides); row offset:
+__host__ __device__ constexpr int bp_row(int d) { return 68 * d - d * (d - 1) / 2; }
+constexpr int BP_TABN = bp_row(BP_MAXD + 1);
 
 struct GeomView {
   const int32_t *off;
@@ -46,6 +58,32 @@ struct GeomView {
 static GeomView view(const Geom *g) {
   return GeomView{g->off, g->seg, g->atn, g->lmax, g->li_z, g->li_sn, g->li_nosi,
                   g->naw, g->nli, g->ndirect, g->nsn, g->vox_z};
+}
+
+// A sub-batch of NB replicates starting at replicate b0 of a batch of nbt:
+// images are [voxel][nbt], sinograms [row][nsn][nbt] (replicate innermost).
+struct Batch {
+  int nbt, b0;
+};
+
+template <int NB>
+__device__ inline void ldv(float (&r)[NB], const float *p) {
+  if constexpr (NB == 1) {
+    r[0] = p[0];
+  } else if constexpr (NB == 2) {
+    float2 t = *reinterpret_cast<const float2 *>(p);
+    r[0] = t.x;
+    r[1] = t.y;
+  } else {
+#pragma unroll
+    for (int k = 0; k < NB; k += 4) {
+      float4 t = *reinterpret_cast<const float4 *>(p + k);
+      r[k] = t.x;
+      r[k + 1] = t.y;
+      r[k + 2] = t.z;
+      r[k + 3] = t.w;
+    }
+  }
 }
 
 // Per-lane state of one Michelogram element along a transaxial bin. The
@@ -74,28 +112,45 @@ __device__ inline void lor_init(Lor &L, float2 z, double atn, double s0) {
   L.ib = wsum - L.ia;
 }
 
-// FP accumulation: pieces go into float partial sums folded into double
-// every SUBSUM segments (repeated additions of nearly equal segment lengths
-// into one float sum round in the same direction and bias the result).
+// FP accumulation: pieces go into float partial sums over SUBSUM segments,
+// folded into the running totals (one long float sum of nearly equal segment
+// lengths would round in the same direction at every step).
+template <int NB>
 struct FpAcc {
-  float pa, pb;
-  double aa, ab;
-  __device__ void piece(const float *col, int ia, int ib, float p, int k) {
-    pa = fmaf(p, col[ia + k], pa);
-    pb = fmaf(p, col[ib - k], pb);
+  float pa[NB], pb[NB], aa[NB], ab[NB];
+  __device__ void zero() {
+#pragma unroll
+    for (int b = 0; b < NB; b++) pa[b] = pb[b] = aa[b] = ab[b] = 0.f;
   }
-  __device__ void direct(const float *col, int ia, float p) { pa = fmaf(p, col[ia], pa); }
+  __device__ void piece(const float *col, int ia, int ib, float p, int k) {
+    float x[NB];
+    ldv<NB>(x, col + (ia + k) * NB);
+#pragma unroll
+    for (int b = 0; b < NB; b++) pa[b] = fmaf(p, x[b], pa[b]);
+    ldv<NB>(x, col + (ib - k) * NB);
+#pragma unroll
+    for (int b = 0; b < NB; b++) pb[b] = fmaf(p, x[b], pb[b]);
+  }
+  __device__ void direct(const float *col, int ia, float p) {
+    float x[NB];
+    ldv<NB>(x, col + ia * NB);
+#pragma unroll
+    for (int b = 0; b < NB; b++) pa[b] = fmaf(p, x[b], pa[b]);
+  }
   __device__ void fold() {
-    aa += (double)pa;
-    ab += (double)pb;
-    pa = 0.f;
-    pb = 0.f;
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      aa[b] += pa[b];
+      ab[b] += pb[b];
+      pa[b] = 0.f;
+      pb[b] = 0.f;
+    }
   }
 };
 
 // One segment with at most one z boundary inside it.
-template <class Op, class F>
-__device__ inline void step1(Lor &l, const SegRec &sg, F *col, Op &op) {
+template <int NB>
+__device__ inline void step1(Lor &l, const SegRec &sg, const float *col, FpAcc<NB> &op) {
   const float q = (float)(l.tb - sg.s);
   const float p1 = __builtin_amdgcn_fmed3f(q, 0.f, sg.len);
   op.piece(col, l.ia, l.ib, p1, 0);
@@ -107,8 +162,8 @@ __device__ inline void step1(Lor &l, const SegRec &sg, F *col, Op &op) {
 }
 
 // One segment with up to two z boundaries inside it.
-template <class Op, class F>
-__device__ inline void step2(Lor &l, const SegRec &sg, F *col, Op &op) {
+template <int NB>
+__device__ inline void step2(Lor &l, const SegRec &sg, const float *col, FpAcc<NB> &op) {
   const float q1 = (float)(l.tb - sg.s);
   const float q2 = (float)(l.tb + l.inv - sg.s);
   const float p1 = __builtin_amdgcn_fmed3f(q1, 0.f, sg.len);
@@ -122,48 +177,27 @@ __device__ inline void step2(Lor &l, const SegRec &sg, F *col, Op &op) {
   l.tb += (double)n * l.inv;
 }
 
-// Walk the CHUNK staged segments for this lane's element of round r.
-template <class Op, class F>
-__device__ inline void walk_chunk(Lor &l, Op &op, const SegRec *sbuf, F *zs, bool direct_wave,
-                                  bool two_wave) {
+// Walk the CH staged segments for this lane's element of one round.
+template <int NB, int CH>
+__device__ inline void walk_chunk(Lor &l, FpAcc<NB> &op, const SegRec *sbuf, const float *zs,
+                                  bool direct_wave, bool two_wave) {
   if (direct_wave) {
-    F *col = zs + l.ia;
+    const float *col = zs + l.ia * NB;
 #pragma unroll
-    for (int j = 0; j < CHUNK; j++) op.direct(col + j * ZSTRIDE, 0, sbuf[j].len);
+    for (int j = 0; j < CH; j++) op.direct(col + j * ZSTRIDE * NB, 0, sbuf[j].len);
     op.fold();
   } else if (two_wave) {
-#pragma unroll 8
-    for (int j = 0; j < CHUNK; j++) {
-      step2(l, sbuf[j], zs + j * ZSTRIDE, op);
+#pragma unroll 4
+    for (int j = 0; j < CH; j++) {
+      step2<NB>(l, sbuf[j], zs + j * ZSTRIDE * NB, op);
       if ((j + 1) % SUBSUM == 0) op.fold();
     }
   } else {
 #pragma unroll
-    for (int j = 0; j < CHUNK; j++) {
-      step1(l, sbuf[j], zs + j * ZSTRIDE, op);
+    for (int j = 0; j < CH; j++) {
+      step1<NB>(l, sbuf[j], zs + j * ZSTRIDE * NB, op);
       if ((j + 1) % SUBSUM == 0) op.fold();
     }
-  }
-}
-
-// Stage the CHUNK segment records of chunk c (padding past the end) and, for
-// FP, their z-columns (BP: zero them).
-__device__ inline void stage_chunk(float *zs, SegRec *sbuf, const GeomView &g, const float *img,
-                                   int beg, int nseg, int c) {
-  const int tid = threadIdx.x;
-  if (tid < CHUNK) {
-    int k = c + tid;
-    SegRec pad;
-    pad.s = SPAD;
-    pad.len = 0.f;
-    pad.colz = 0;
-    sbuf[tid] = (k < nseg) ? g.seg[beg + k] : pad;
-  }
-  for (int idx = tid; idx < CHUNK * ZSTRIDE; idx += PRJ_THREADS) {
-    int j = idx >> 7, z = idx & (ZSTRIDE - 1), k = c + j;
-    float v = 0.f;
-    if (img && k < nseg && z < NZ) v = img[g.seg[beg + k].colz + z];
-    zs[idx] = v;
   }
 }
 
@@ -173,59 +207,78 @@ __device__ inline float lor_sfac(float dz, float vox_z, double atn) {
   return sqrtf(1.f + r * r);
 }
 
+__device__ inline SegRec pad_seg() {
+  SegRec p;
+  p.s = SPAD;
+  p.len = 0.f;
+  p.colz = 0;
+  return p;
+}
+
 // ---------------------------------------------------------------------------
 // Forward projection. MODE 0: emission; 1: attenuation factors;
 // 2: OSEM ratio y / (FP + rho) with the 1e-6 guards of prj/src/recon.cu:252-283.
-template <int MODE>
+// rho_nb: 0 = one rho shared by all replicates, else rho has the batch layout.
+template <int MODE, int NB>
 __global__ __launch_bounds__(PRJ_THREADS) void k_fprj(GeomView g, const float *__restrict__ img,
                                                       float *__restrict__ sino,
                                                       const int32_t *__restrict__ subs,
                                                       const uint16_t *__restrict__ prompts,
-                                                      const float *__restrict__ rho) {
-  __shared__ float zbuf[ZGUARD + CHUNK * ZSTRIDE + ZGUARD];
-  __shared__ SegRec sbuf[CHUNK];
-  __shared__ float bins[MAXNSN];
-  float *zs = zbuf + ZGUARD;
+                                                      const float *__restrict__ rho, int rho_nb,
+                                                      Batch bt) {
+  constexpr int CH = fp_chunk<NB>();
+  __shared__ __align__(16) float zbuf[(ZGUARD + CH * ZSTRIDE + ZGUARD) * NB];
+  __shared__ SegRec sbuf[CH];
+  __shared__ float bins[fp_bins<NB>()];
+  float *zs = zbuf + ZGUARD * NB;
   const int tid = threadIdx.x, wave = tid >> 6;
   const int row = blockIdx.x;
   const int aw = subs ? subs[row] : row;
   const int nsn = g.nsn;
-  float *out = sino + (size_t)row * nsn;
+  const int nbt = bt.nbt, b0 = bt.b0;
+  float *out = sino + (size_t)row * nsn * nbt + b0;
   if (aw < 0 || aw >= g.naw) {  // invalid bin index: zero row
-    for (int i = tid; i < nsn; i += PRJ_THREADS) out[i] = 0.f;
+    for (int i = tid; i < nsn * NB; i += PRJ_THREADS) out[(i / NB) * nbt + i % NB] = 0.f;
     return;
   }
   const int beg = g.off[aw], nseg = g.off[aw + 1] - beg;
   const double atn = g.atn[aw];
   const float lmax = g.lmax[aw];
-  for (int i = tid; i < nsn; i += PRJ_THREADS) bins[i] = 0.f;
-  if (tid < ZGUARD) {
+  for (int i = tid; i < nsn * NB; i += PRJ_THREADS) bins[i] = 0.f;
+  if (tid < ZGUARD * NB) {
     zbuf[tid] = 0.f;
-    zbuf[ZGUARD + CHUNK * ZSTRIDE + tid] = 0.f;
+    zbuf[(ZGUARD + CH * ZSTRIDE) * NB + tid] = 0.f;
   }
   const double s0 = nseg > 0 ? g.seg[beg].s : 0.0;
 
   Lor L[LI_ROUNDS];
-  FpAcc A[LI_ROUNDS];
+  FpAcc<NB> A[LI_ROUNDS];
   bool two[LI_ROUNDS];
 #pragma unroll
   for (int r = 0; r < LI_ROUNDS; r++) {
     int li = r * PRJ_THREADS + tid;
     float2 z = li < g.nli ? g.li_z[li] : make_float2(0.5f, 0.f);
     lor_init(L[r], z, atn, s0);
-    A[r] = FpAcc{0.f, 0.f, 0.0, 0.0};
+    A[r].zero();
     two[r] = __any(L[r].inv > 0.0 && L[r].inv < (double)lmax);
   }
 
-  for (int c = 0; c < nseg; c += CHUNK) {
+  for (int c = 0; c < nseg; c += CH) {
     __syncthreads();
-    stage_chunk(zs, sbuf, g, img, beg, nseg, c);
+    if (tid < CH) sbuf[tid] = (c + tid < nseg) ? g.seg[beg + c + tid] : pad_seg();
+    for (int idx = tid; idx < CH * ZSTRIDE * NB; idx += PRJ_THREADS) {
+      const int j = idx / (ZSTRIDE * NB), rem = idx - j * (ZSTRIDE * NB);
+      const int z = rem / NB, b = rem - z * NB, k = c + j;
+      float v = 0.f;
+      if (k < nseg && z < NZ) v = img[((int64_t)g.seg[beg + k].colz + z) * nbt + b0 + b];
+      zs[idx] = v;
+    }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < LI_ROUNDS; r++) {
       const int li0 = r * PRJ_THREADS + wave * 64;  // wave-uniform
       if (li0 >= g.nli) continue;
-      walk_chunk(L[r], A[r], sbuf, (const float *)zs, li0 + 64 <= g.ndirect, two[r]);
+      walk_chunk<NB, CH>(L[r], A[r], sbuf, zs, li0 + 64 <= g.ndirect, two[r]);
     }
   }
 
@@ -234,28 +287,33 @@ __global__ __launch_bounds__(PRJ_THREADS) void k_fprj(GeomView g, const float *_
   for (int r = 0; r < LI_ROUNDS; r++) {
     int li = r * PRJ_THREADS + tid;
     if (li < g.nli) {
+      A[r].fold();
       float2 z = g.li_z[li];
       short2 sn = g.li_sn[li];
       float sf = lor_sfac(z.y, g.vox_z, atn);
-      float va = (float)A[r].aa * sf, vb = (float)A[r].ab * sf;
-      if (MODE == 1) {
-        float nosi = g.li_nosi[li];
-        va = expf(-va) * nosi;
-        vb = expf(-vb) * nosi;
+      float nosi = g.li_nosi[li];
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        float va = A[r].aa[b] * sf, vb = A[r].ab[b] * sf;
+        if (MODE == 1) {
+          va = expf(-va) * nosi;
+          vb = expf(-vb) * nosi;
+        }
+        atomicAdd(&bins[sn.x * NB + b], va);
+        if (z.y > 0.f) atomicAdd(&bins[sn.y * NB + b], vb);
       }
-      atomicAdd(&bins[sn.x], va);
-      if (z.y > 0.f) atomicAdd(&bins[sn.y], vb);
     }
   }
   __syncthreads();
-  for (int i = tid; i < nsn; i += PRJ_THREADS) {
+  for (int i = tid; i < nsn * NB; i += PRJ_THREADS) {
+    const int s = i / NB, b = i - s * NB;
     float v = bins[i];
     if (MODE == 2) {
-      size_t k = (size_t)aw * nsn + i;
-      float e = v + rho[k];
-      v = (e > -EPS && e < EPS) ? 0.f : (float)prompts[k] / e;
+      const size_t k = (size_t)aw * nsn + s;
+      const float e = v + (rho_nb ? rho[k * nbt + b0 + b] : rho[k]);
+      v = (e > -EPS && e < EPS) ? 0.f : (float)prompts[k * nbt + b0 + b] / e;
     }
-    out[i] = v;
+    out[(size_t)s * nbt + b] = v;
   }
 }
 
@@ -272,74 +330,71 @@ __global__ __launch_bounds__(PRJ_THREADS) void k_fprj(GeomView g, const float *_
 //   w0 odd:  2q   <- p2 (Y + Xb),   2q+1 <- p1 (X + Xb) + p3 (Y + Yb)
 // with X = va[q - h], Y = va[q - h - 1], Xb = vb[q - e], Yb = vb[q - e + 1],
 // h = w0 >> 1, e = (2d - w0) >> 1 (va/vb: the bin values of the LORs and of
-// their mirrors times the 3D length factor, by r0). The pieces of (segment, d)
-// are computed by lane d and broadcast with readlane. No atomics inside a ray;
-// one coalesced global float atomic per (segment, plane) merges rays.
-constexpr int BP_ROW = 72;          // LDS row of va / vb: r0 = -4 .. 67
-constexpr int BP_OFF = 4;           // r0 = 0 sits at BP_OFF
-constexpr int BP_MAXD = 64;         // ring differences handled (0 .. 63)
-
+// their mirrors times the 3D length factor, by r0; rows padded with zeros so
+// that clamped indices read zeros). The pieces of (segment, d) are computed by
+// lane d and broadcast with readlane. No atomics inside a ray; one coalesced
+// global float atomic per (segment, plane, replicate) merges rays.
 __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 __device__ inline float rdlane(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
-__global__ __launch_bounds__(PRJ_THREADS) void k_bprj(GeomView g, float *__restrict__ img,
-                                                      const float *__restrict__ sino,
-                                                      const int32_t *__restrict__ subs) {
-  __shared__ float va[BP_MAXD * BP_ROW];
-  __shared__ float vb[BP_MAXD * BP_ROW];
-  __shared__ float zs[CHUNK * ZSTRIDE];
-  __shared__ SegRec sbuf[CHUNK];
+template <int NB>
+__global__ __launch_bounds__(BP_THREADS) void k_bprj(GeomView g, float *__restrict__ img,
+                                                     const float *__restrict__ sino,
+                                                     const int32_t *__restrict__ subs, Batch bt) {
+  constexpr int CH = bp_chunk<NB>();
+  __shared__ __align__(16) float va[BP_TABN * NB];
+  __shared__ __align__(16) float vb[BP_TABN * NB];
+  __shared__ float zs[CH * ZSTRIDE * NB];
+  __shared__ SegRec sbuf[CH];
   __shared__ int dmax_s;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int row = blockIdx.x;
   const int aw = subs ? subs[row] : row;
   if (aw < 0 || aw >= g.naw) return;
-  const int nsn = g.nsn;
-  const float *in = sino + (size_t)row * nsn;
+  const int nsn = g.nsn, nbt = bt.nbt, b0 = bt.b0;
+  const float *in = sino + (size_t)row * nsn * nbt + b0;
   const int beg = g.off[aw], nseg = g.off[aw + 1] - beg;
   const double atn = g.atn[aw];
 
-  // va / vb tables: zero, then the LOR values (bin value * 3D length factor)
-  for (int i = tid; i < BP_MAXD * BP_ROW; i += PRJ_THREADS) {
+  for (int i = tid; i < BP_TABN * NB; i += BP_THREADS) {
     va[i] = 0.f;
     vb[i] = 0.f;
   }
   if (tid == 0) dmax_s = 0;
   __syncthreads();
-  for (int li = tid; li < g.nli; li += PRJ_THREADS) {
+  for (int i = tid; i < g.nli * NB; i += BP_THREADS) {
+    const int li = i / NB, b = i - li * NB;
     float2 z = g.li_z[li];
     short2 sn = g.li_sn[li];
     int r0 = (int)floorf(z.x) >> 1, d = (int)z.y >> 1;
     float sf = lor_sfac(z.y, g.vox_z, atn);
-    int k = d * BP_ROW + BP_OFF + r0;
-    va[k] = in[sn.x] * sf;
-    if (d > 0) vb[k] = in[sn.y] * sf;
-    atomicMax(&dmax_s, d);
+    int k = (bp_row(d) + 2 + r0) * NB + b;
+    va[k] = in[sn.x * nbt + b] * sf;
+    if (d > 0) vb[k] = in[sn.y * nbt + b] * sf;
+    if (b == 0) atomicMax(&dmax_s, d);
   }
   __syncthreads();
   const int dmax = dmax_s;
-  // per-lane constant: d = lane, slope 2d/atn of zeta along t
+  // per-lane constants: d = lane, slope 2d/atn of zeta along t
   const double m_lane = 2.0 * lane / atn;
   const double inv_lane = lane > 0 ? atn / (2.0 * lane) : 0.0;
   const int q = lane;
 
-  for (int c = 0; c < nseg; c += CHUNK) {
+  for (int c = 0; c < nseg; c += CH) {
     __syncthreads();
-    if (tid < CHUNK) {
-      int k = c + tid;
-      SegRec pad;
-      pad.s = 0.0;
-      pad.len = 0.f;
-      pad.colz = 0;
-      sbuf[tid] = (k < nseg) ? g.seg[beg + k] : pad;
+    if (tid < CH) {
+      SegRec z;
+      z.s = 0.0;
+      z.len = 0.f;
+      z.colz = 0;
+      sbuf[tid] = (c + tid < nseg) ? g.seg[beg + c + tid] : z;
     }
     __syncthreads();
-    for (int j = wave; j < CHUNK; j += PRJ_THREADS / 64) {
+    for (int j = wave; j < CH; j += BP_THREADS / 64) {
       const SegRec sg = sbuf[j];
-      // lane d: base plane and pieces of ring difference d on this segment
       float P1 = 0.f, P2 = 0.f, P3 = 0.f;
       int W0 = 0;
       if (lane >= 1 && lane <= dmax) {
@@ -353,38 +408,56 @@ __global__ __launch_bounds__(PRJ_THREADS) void k_bprj(GeomView g, float *__restr
         P3 = sg.len - c2;
         W0 = (int)w0;
       }
-      // direct planes: plane 2q <- L * va0[q]
-      float acc0 = sg.len * va[BP_OFF + q], acc1 = 0.f;
+      float acc0[NB], acc1[NB], X[NB], Y[NB], Xb[NB], Yb[NB];
+      ldv<NB>(X, va + (bp_row(0) + 2 + q) * NB);  // direct planes: plane 2q <- L va0[q]
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        acc0[b] = sg.len * X[b];
+        acc1[b] = 0.f;
+      }
       for (int d = 1; d <= dmax; d++) {
         const float p1 = rdlane(P1, d), p2 = rdlane(P2, d), p3 = rdlane(P3, d);
         const int w0 = __builtin_amdgcn_readlane(W0, d);
         const int h = w0 >> 1, e = (2 * d - w0) >> 1;
-        const float *ra = va + d * BP_ROW + BP_OFF;
-        const float *rb = vb + d * BP_ROW + BP_OFF;
-        // clamped indices always land on zero entries (r0 < 0 or r0 > 63 - d)
-        const int ia = clampi(q - h, -3, 64);
-        const int ib = clampi(q - e, -3, 64);
-        const float X = ra[ia], Y = ra[ia - 1], Xb = rb[ib], Yb = rb[ib + 1];
+        const int rb = bp_row(d) + 2;
+        // clamped indices always land on zero pads (r0 < 0 or r0 > 63 - d)
+        const int ia = rb + clampi(q - h, -1, 65 - d);
+        const int ib = rb + clampi(q - e, -2, 64 - d);
+        ldv<NB>(X, va + ia * NB);
+        ldv<NB>(Y, va + (ia - 1) * NB);
+        ldv<NB>(Xb, vb + ib * NB);
+        ldv<NB>(Yb, vb + (ib + 1) * NB);
         if (w0 & 1) {
-          acc0 = fmaf(p2, Y + Xb, acc0);
-          acc1 = fmaf(p1, X + Xb, acc1);
-          acc1 = fmaf(p3, Y + Yb, acc1);
+#pragma unroll
+          for (int b = 0; b < NB; b++) {
+            acc0[b] = fmaf(p2, Y[b] + Xb[b], acc0[b]);
+            acc1[b] = fmaf(p1, X[b] + Xb[b], acc1[b]);
+            acc1[b] = fmaf(p3, Y[b] + Yb[b], acc1[b]);
+          }
         } else {
-          acc0 = fmaf(p1, X + Xb, acc0);
-          acc0 = fmaf(p3, Y + Yb, acc0);
-          acc1 = fmaf(p2, X + Yb, acc1);
+#pragma unroll
+          for (int b = 0; b < NB; b++) {
+            acc0[b] = fmaf(p1, X[b] + Xb[b], acc0[b]);
+            acc0[b] = fmaf(p3, Y[b] + Yb[b], acc0[b]);
+            acc1[b] = fmaf(p2, X[b] + Yb[b], acc1[b]);
+          }
         }
       }
-      zs[j * ZSTRIDE + 2 * q] = acc0;
-      zs[j * ZSTRIDE + 2 * q + 1] = acc1;
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        zs[(j * ZSTRIDE + 2 * q) * NB + b] = acc0[b];
+        zs[(j * ZSTRIDE + 2 * q + 1) * NB + b] = acc1[b];
+      }
     }
     __syncthreads();
-    // flush the chunk: one coalesced global atomic per (segment, plane)
-    const int nj = min(CHUNK, nseg - c);
-    for (int idx = tid; idx < nj * ZSTRIDE; idx += PRJ_THREADS) {
-      int j = idx >> 7, z = idx & (ZSTRIDE - 1);
-      float v = zs[idx];
-      if (z < NZ && v != 0.f) unsafeAtomicAdd(img + sbuf[j].colz + z, v);
+    // flush the chunk: one coalesced global atomic per (segment, plane, replicate)
+    const int nj = min(CH, nseg - c);
+    for (int idx = tid; idx < nj * ZSTRIDE * NB; idx += BP_THREADS) {
+      const int j = idx / (ZSTRIDE * NB), rem = idx - j * (ZSTRIDE * NB);
+      const int z = rem / NB, b = rem - z * NB;
+      const float v = zs[idx];
+      if (z < NZ && v != 0.f)
+        unsafeAtomicAdd(img + ((int64_t)sbuf[j].colz + z) * nbt + b0 + b, v);
     }
   }
 }
@@ -411,15 +484,18 @@ __global__ void k_osem_corr(float *rho, float *an, const float *a, const float *
 }
 
 // img = mask ? img * (b / sens) : 0 with b/sens = 0 where |sens| < eps
-// (prj/src/recon.cu:234-247, 300-315).
+// (prj/src/recon.cu:234-247, 300-315); images [voxel][nb], sens shared
+// (sens_nb = 0) or [voxel][nb].
 __global__ void k_osem_update(float *img, const float *b, const float *sens, const uint8_t *mask,
-                              int64_t n) {
+                              int64_t nvox, int nb, int sens_nb) {
+  const int64_t n = nvox * nb;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) {
-    float sv = sens[i];
+    const int64_t v = i / nb;
+    float sv = sens_nb ? sens[i] : sens[v];
     float q = (sv > -EPS && sv < EPS) ? 0.f : b[i] / sv;
-    img[i] = mask[i] ? img[i] * q : 0.f;
+    img[i] = mask[v] ? img[i] * q : 0.f;
   }
 }
 
@@ -459,6 +535,48 @@ static int ew_blocks(int64_t n) {
 
 using namespace nipet;
 
+// Launch a projector over nb replicates as sub-batches of 4, 2 and 1.
+template <int MODE>
+static void launch_fprj(const Geom *g, float *sino, const float *img, const int32_t *subs,
+                        int nprj, const uint16_t *prompts, const float *rho, int rho_nb, int nb,
+                        hipStream_t st) {
+  for (int b0 = 0; b0 < nb;) {
+    const int rem = nb - b0;
+    const Batch bt{nb, b0};
+    if (rem >= 4) {
+      k_fprj<MODE, 4><<<nprj, PRJ_THREADS, 0, st>>>(view(g), img, sino, subs, prompts, rho,
+                                                   rho_nb, bt);
+      b0 += 4;
+    } else if (rem >= 2) {
+      k_fprj<MODE, 2><<<nprj, PRJ_THREADS, 0, st>>>(view(g), img, sino, subs, prompts, rho,
+                                                   rho_nb, bt);
+      b0 += 2;
+    } else {
+      k_fprj<MODE, 1><<<nprj, PRJ_THREADS, 0, st>>>(view(g), img, sino, subs, prompts, rho,
+                                                   rho_nb, bt);
+      b0 += 1;
+    }
+  }
+}
+
+static void launch_bprj(const Geom *g, float *img, const float *sino, const int32_t *subs,
+                        int nprj, int nb, hipStream_t st) {
+  for (int b0 = 0; b0 < nb;) {
+    const int rem = nb - b0;
+    const Batch bt{nb, b0};
+    if (rem >= 4) {
+      k_bprj<4><<<nprj, BP_THREADS, 0, st>>>(view(g), img, sino, subs, bt);
+      b0 += 4;
+    } else if (rem >= 2) {
+      k_bprj<2><<<nprj, BP_THREADS, 0, st>>>(view(g), img, sino, subs, bt);
+      b0 += 2;
+    } else {
+      k_bprj<1><<<nprj, BP_THREADS, 0, st>>>(view(g), img, sino, subs, bt);
+      b0 += 1;
+    }
+  }
+}
+
 static int check_geom(const NipetGeom *g, const char *fn) {
   if (!g) {
     set_error("%s: NULL geometry", fn);
@@ -472,9 +590,17 @@ static int check_geom(const NipetGeom *g, const char *fn) {
   return NIPET_OK;
 }
 
-static int check_nprj(const NipetGeom *g, const int32_t *subs, int nprj, const char *fn) {
+static int check_nprj(const NipetGeom *g, const int32_t *subs, int nprj, int nb, const char *fn) {
   if (nprj <= 0 || (!subs && nprj != g->naw)) {
     set_error("%s: nprj=%d (must be > 0, and = naw=%d when subs is NULL)", fn, nprj, g->naw);
+    return NIPET_EINVAL;
+  }
+  if (nb < 1 || nb > NIPET_MAXBATCH) {
+    set_error("%s: batch %d outside 1..%d", fn, nb, NIPET_MAXBATCH);
+    return NIPET_EINVAL;
+  }
+  if (nb > 1 && g->nsn > 837) {
+    set_error("%s: batches need at most 837 sinograms (span 11)", fn);
     return NIPET_EINVAL;
   }
   return NIPET_OK;
@@ -482,11 +608,11 @@ static int check_nprj(const NipetGeom *g, const int32_t *subs, int nprj, const c
 
 extern "C" {
 
-int nipet_fprj(const NipetGeom *g, float *sino, const float *img, const int32_t *subs, int nprj,
-               int att, void *stream) {
+int nipet_fprj_b(const NipetGeom *g, float *sino, const float *img, const int32_t *subs,
+                 int nprj, int nb, int att, void *stream) {
   int rc = check_geom(g, "nipet_fprj");
   if (rc) return rc;
-  if ((rc = check_nprj(g, subs, nprj, "nipet_fprj"))) return rc;
+  if ((rc = check_nprj(g, subs, nprj, nb, "nipet_fprj"))) return rc;
   if (!sino || !img || (att != 0 && att != 1)) {
     set_error("nipet_fprj: NULL buffer or att not in {0,1}");
     return NIPET_EINVAL;
@@ -494,32 +620,42 @@ int nipet_fprj(const NipetGeom *g, float *sino, const float *img, const int32_t 
   hipStream_t st = (hipStream_t)stream;
   prof_begin(PK_FPRJ, st);
   if (att)
-    k_fprj<1><<<nprj, PRJ_THREADS, 0, st>>>(view(g), img, sino, subs, nullptr, nullptr);
+    launch_fprj<1>(g, sino, img, subs, nprj, nullptr, nullptr, 0, nb, st);
   else
-    k_fprj<0><<<nprj, PRJ_THREADS, 0, st>>>(view(g), img, sino, subs, nullptr, nullptr);
+    launch_fprj<0>(g, sino, img, subs, nprj, nullptr, nullptr, 0, nb, st);
   prof_end(PK_FPRJ, st);
   NIPET_HIP(hipGetLastError());
   return NIPET_OK;
 }
 
-int nipet_bprj(const NipetGeom *g, float *img, const float *sino, const int32_t *subs, int nprj,
-               void *stream) {
+int nipet_fprj(const NipetGeom *g, float *sino, const float *img, const int32_t *subs, int nprj,
+               int att, void *stream) {
+  return nipet_fprj_b(g, sino, img, subs, nprj, 1, att, stream);
+}
+
+int nipet_bprj_b(const NipetGeom *g, float *img, const float *sino, const int32_t *subs,
+                 int nprj, int nb, void *stream) {
   int rc = check_geom(g, "nipet_bprj");
   if (rc) return rc;
-  if ((rc = check_nprj(g, subs, nprj, "nipet_bprj"))) return rc;
+  if ((rc = check_nprj(g, subs, nprj, nb, "nipet_bprj"))) return rc;
   if (!sino || !img) {
     set_error("nipet_bprj: NULL buffer");
     return NIPET_EINVAL;
   }
   hipStream_t st = (hipStream_t)stream;
   prof_begin(PK_FILL, st);
-  k_fill<<<ew_blocks(NVOX), 256, 0, st>>>(img, 0.f, NVOX);
+  k_fill<<<ew_blocks(NVOX * nb), 256, 0, st>>>(img, 0.f, NVOX * nb);
   prof_end(PK_FILL, st);
   prof_begin(PK_BPRJ, st);
-  k_bprj<<<nprj, PRJ_THREADS, 0, st>>>(view(g), img, sino, subs);
+  launch_bprj(g, img, sino, subs, nprj, nb, st);
   prof_end(PK_BPRJ, st);
   NIPET_HIP(hipGetLastError());
   return NIPET_OK;
+}
+
+int nipet_bprj(const NipetGeom *g, float *img, const float *sino, const int32_t *subs, int nprj,
+               void *stream) {
+  return nipet_bprj_b(g, img, sino, subs, nprj, 1, stream);
 }
 
 int nipet_remgaps(float *out, const void *in, int in_dtype, const int32_t *aw2ali, int naw,
@@ -566,34 +702,44 @@ int nipet_osem_corr(float *rho, float *an, const float *a, const float *n, const
   return NIPET_OK;
 }
 
-int nipet_osem_subset(const NipetGeom *g, float *img, const uint16_t *prompts, const float *rho,
-                      const float *sens, const uint8_t *mask, const int32_t *subs, int nprj,
-                      float *work_sino, float *work_img, void *stream) {
+int nipet_osem_subset_b(const NipetGeom *g, float *img, const uint16_t *prompts,
+                        const float *rho, int rho_per_rep, const float *sens, int sens_per_rep,
+                        const uint8_t *mask, const int32_t *subs, int nprj, int nb,
+                        float *work_sino, float *work_img, void *stream) {
   int rc = check_geom(g, "nipet_osem_subset");
   if (rc) return rc;
   if (!subs || nprj <= 0 || nprj > g->naw) {
     set_error("nipet_osem_subset: subs must be given with 0 < nprj <= naw");
     return NIPET_EINVAL;
   }
+  if ((rc = check_nprj(g, subs, nprj, nb, "nipet_osem_subset"))) return rc;
   if (!img || !prompts || !rho || !sens || !mask || !work_sino || !work_img) {
     set_error("nipet_osem_subset: NULL buffer");
     return NIPET_EINVAL;
   }
   hipStream_t st = (hipStream_t)stream;
   prof_begin(PK_FPRJ, st);
-  k_fprj<2><<<nprj, PRJ_THREADS, 0, st>>>(view(g), img, work_sino, subs, prompts, rho);
+  launch_fprj<2>(g, work_sino, img, subs, nprj, prompts, rho, rho_per_rep ? 1 : 0, nb, st);
   prof_end(PK_FPRJ, st);
   prof_begin(PK_FILL, st);
-  k_fill<<<ew_blocks(NVOX), 256, 0, st>>>(work_img, 0.f, NVOX);
+  k_fill<<<ew_blocks(NVOX * nb), 256, 0, st>>>(work_img, 0.f, NVOX * nb);
   prof_end(PK_FILL, st);
   prof_begin(PK_BPRJ, st);
-  k_bprj<<<nprj, PRJ_THREADS, 0, st>>>(view(g), work_img, work_sino, subs);
+  launch_bprj(g, work_img, work_sino, subs, nprj, nb, st);
   prof_end(PK_BPRJ, st);
   prof_begin(PK_OSEM_UPDATE, st);
-  k_osem_update<<<ew_blocks(NVOX), 256, 0, st>>>(img, work_img, sens, mask, NVOX);
+  k_osem_update<<<ew_blocks(NVOX * nb), 256, 0, st>>>(img, work_img, sens, mask, NVOX, nb,
+                                                       sens_per_rep ? 1 : 0);
   prof_end(PK_OSEM_UPDATE, st);
   NIPET_HIP(hipGetLastError());
   return NIPET_OK;
+}
+
+int nipet_osem_subset(const NipetGeom *g, float *img, const uint16_t *prompts, const float *rho,
+                      const float *sens, const uint8_t *mask, const int32_t *subs, int nprj,
+                      float *work_sino, float *work_img, void *stream) {
+  return nipet_osem_subset_b(g, img, prompts, rho, 0, sens, 0, mask, subs, nprj, 1, work_sino,
+                             work_img, stream);
 }
 
 }  // extern "C"

@@ -120,3 +120,98 @@ def osem_run(state, niter, img0=None, check_counts=True):
         if check_counts and float(torch.nansum(img)) < 0.1:
             break
     return img
+
+
+class OsemBatch:
+    """Device state of nb reconstructions that share geometry, attenuation x
+    norm (hence sensitivity images) and mask: bootstrap replicates of one frame
+    (shared rho) or frames of one scan (rho per frame). Images are
+    [320, 320, 127, nb] (replicate innermost)."""
+
+    def __init__(self, geom, subsets, prompts, rho, rho_per_rep, sens, mask, nb):
+        torch = _torch()
+        self.geom, self.subsets, self.nb = geom, subsets, nb
+        self.prompts, self.rho, self.rho_per_rep, self.sens, self.mask = (
+            prompts, rho, rho_per_rep, sens, mask)
+        self.nprj = max(len(s) for s in subsets)
+        self.work_sino = torch.empty((self.nprj, geom.nsn, nb), dtype=torch.float32,
+                                     device='cuda')
+        self.work_img = torch.empty((320, 320, 127, nb), dtype=torch.float32, device='cuda')
+
+
+def osem_prepare_batch(scanner_params, prompts, asng, nsng, rsng=None, ssng=None,
+                       mask_radius=29., subsets=None, mask=None, sens=None):
+    """Batch state: ``prompts`` (Naw, nsn, nb) (uint16 or counts), shared
+    attenuation / norm (Naw, nsn); randoms / scatter either shared (Naw, nsn) or
+    per replicate (Naw, nsn, nb). ``sens`` may be passed from an earlier batch
+    with the same a n (a (14, 320, 320, 127) tensor)."""
+    torch = _torch()
+    from ..img.mmrimg import get_cylinder
+    from .mmrrec import get_subsets14
+    geom = get_geom(scanner_params)
+    Cnt = scanner_params['Cnt']
+    p = prompts if isinstance(prompts, torch.Tensor) else torch.from_numpy(np.asarray(prompts))
+    p = p.to('cuda')
+    if p.dtype != torch.uint16:
+        p = p.to(torch.int32).clamp(0, 65535).to(torch.uint16)
+    p = p.contiguous()
+    if p.dim() != 3 or tuple(p.shape[:2]) != (geom.naw, geom.nsn):
+        raise ValueError(f'prompts must be (Naw, nsn, nb) = ({geom.naw}, {geom.nsn}, nb)')
+    nb = p.shape[2]
+    a = _dev(asng, torch.float32)
+    n = _dev(nsng, torch.float32)
+    per_rep = any(x is not None and tuple(x.shape) == (geom.naw, geom.nsn, nb)
+                  for x in (rsng, ssng))
+    shape = (geom.naw, geom.nsn, nb) if per_rep else (geom.naw, geom.nsn)
+
+    def add(x):
+        if x is None:
+            return None
+        x = _dev(x, torch.float32)
+        return x if x.dim() == len(shape) else x[..., None].expand(shape).contiguous()
+
+    r, s = add(rsng), add(ssng)
+    rs = None if (r is None and s is None) else ((r if r is not None else 0) +
+                                                   (s if s is not None else 0)).contiguous()
+    an = (a * n).contiguous()
+    anb = an if not per_rep else an[..., None].expand(shape).contiguous()
+    rho = torch.empty(shape, dtype=torch.float32, device='cuda')
+    tmp = torch.empty(shape, dtype=torch.float32, device='cuda')
+    one = torch.ones(shape, dtype=torch.float32, device='cuda')
+    _lib.check(_lib.lib().nipet_osem_corr(_lib.ptr(rho), _lib.ptr(tmp), _lib.ptr(anb),
+                                          _lib.ptr(one), _lib.ptr(rs), None, rho.numel(),
+                                          _lib.stream_handle()), 'nipet_osem_corr')
+    del tmp, one
+    if subsets is None:
+        subsets = [get_subsets14(k, scanner_params)[0] for k in range(14)]
+    subs = [x.to('cuda', torch.int32).contiguous() if isinstance(x, torch.Tensor) else
+            _dev(np.asarray(x, np.int32), torch.int32) for x in subsets]
+    if sens is None:
+        sens = torch.stack([sensitivity(geom, an, x) for x in subs])
+    if mask is None:
+        mask = get_cylinder(Cnt, rad=mask_radius, xo=0, yo=0, unival=1, gpu_dim=True) > 0.9
+    msk = _dev(mask if isinstance(mask, torch.Tensor) else np.asarray(mask, np.uint8),
+               torch.uint8)
+    return OsemBatch(geom, subs, p, rho, per_rep, sens, msk, nb)
+
+
+def osem_iteration_batch(state, img):
+    st = _lib.stream_handle()
+    L = _lib.lib()
+    for k, subs in enumerate(state.subsets):
+        _lib.check(L.nipet_osem_subset_b(
+            state.geom.handle, _lib.ptr(img), _lib.ptr(state.prompts), _lib.ptr(state.rho),
+            int(state.rho_per_rep), _lib.ptr(state.sens[k]), 0, _lib.ptr(state.mask),
+            _lib.ptr(subs), len(subs), state.nb, _lib.ptr(state.work_sino),
+            _lib.ptr(state.work_img), st), 'nipet_osem_subset_b')
+
+
+def osem_run_batch(state, niter, img0=None):
+    """``niter`` OSEM iterations of all nb images (ones by default); returns
+    [320, 320, 127, nb]."""
+    torch = _torch()
+    img = torch.ones((320, 320, 127, state.nb), dtype=torch.float32, device='cuda') \
+        if img0 is None else _dev(img0, torch.float32).clone()
+    for _ in range(niter):
+        osem_iteration_batch(state, img)
+    return img
