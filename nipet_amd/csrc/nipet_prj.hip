@@ -329,11 +329,17 @@ __global__ __launch_bounds__(PRJ_THREADS) void k_fprj(GeomView g, const float *_
 //   w0 even: 2q   <- p1 (X + Xb) + p3 (Y + Yb),   2q+1 <- p2 (X + Yb)
 //   w0 odd:  2q   <- p2 (Y + Xb),   2q+1 <- p1 (X + Xb) + p3 (Y + Yb)
 // with X = va[q - h], Y = va[q - h - 1], Xb = vb[q - e], Yb = vb[q - e + 1],
-// h = w0 >> 1, e = (2d - w0) >> 1 (va/vb: the bin values of the LORs and of
-// their mirrors times the 3D length factor, by r0; rows padded with zeros so
-// that clamped indices read zeros). The pieces of (segment, d) are computed by
-// lane d and broadcast with readlane. No atomics inside a ray; one coalesced
-// global float atomic per (segment, plane, replicate) merges rays.
+// h = w0 >> 1, e = (2d - w0) >> 1 (va / vb: sinogram values of the LORs
+// (r0, d) and of their mirrors). Each wave takes runs of SPW consecutive
+// segments and walks all ring differences; for a given d, w0 changes between
+// consecutive segments only when a z boundary lies between their starts, so
+// the table reads and sums are kept in registers and reloaded only on change.
+// The pieces of (segment, d) are computed by lane d (times the 3D length
+// factor of d) and broadcast with readlane. va / vb are read from an LDS copy
+// of the sinogram row through per-(d, r0) index tables whose padding entries
+// point at a zero row, so clamped indices read zeros. No atomics inside a ray:
+// each wave flushes its segments through a private LDS buffer with one
+// coalesced global float atomic per (segment, plane, replicate).
 __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 __device__ inline float rdlane(float v, int l) {
@@ -341,14 +347,18 @@ __device__ inline float rdlane(float v, int l) {
 }
 
 template <int NB>
+constexpr int bp_spw() { return NB == 1 ? 8 : 4; }
+
+template <int NB>
 __global__ __launch_bounds__(BP_THREADS) void k_bprj(GeomView g, float *__restrict__ img,
                                                      const float *__restrict__ sino,
                                                      const int32_t *__restrict__ subs, Batch bt) {
-  constexpr int CH = bp_chunk<NB>();
-  __shared__ __align__(16) float va[BP_TABN * NB];
-  __shared__ __align__(16) float vb[BP_TABN * NB];
-  __shared__ float zs[CH * ZSTRIDE * NB];
-  __shared__ SegRec sbuf[CH];
+  constexpr int SPW = bp_spw<NB>();
+  constexpr int NWAVE = BP_THREADS / 64;
+  __shared__ __align__(16) float srow[(MAXNSN / NB + 1) * NB];   // sinogram row + zero row
+  __shared__ int32_t pa[BP_TABN];  // (idx_a[k-1] << 16) | idx_a[k]
+  __shared__ int32_t pb[BP_TABN];  // (idx_b[k+1] << 16) | idx_b[k]
+  __shared__ __align__(16) float wbuf[NWAVE][ZSTRIDE * NB];
   __shared__ int dmax_s;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int row = blockIdx.x;
@@ -359,105 +369,157 @@ __global__ __launch_bounds__(BP_THREADS) void k_bprj(GeomView g, float *__restri
   const int beg = g.off[aw], nseg = g.off[aw + 1] - beg;
   const double atn = g.atn[aw];
 
-  for (int i = tid; i < BP_TABN * NB; i += BP_THREADS) {
-    va[i] = 0.f;
-    vb[i] = 0.f;
+  // LDS: sinogram row, zero row at index nsn, LOR -> sinogram index tables
+  for (int i = tid; i < (nsn + 1) * NB; i += BP_THREADS) {
+    const int s = i / NB, b = i - s * NB;
+    srow[i] = s < nsn ? in[(size_t)s * nbt + b] : 0.f;
+  }
+  __shared__ int16_t ia_tab[BP_TABN], ib_tab[BP_TABN];
+  for (int i = tid; i < BP_TABN; i += BP_THREADS) {
+    ia_tab[i] = (int16_t)nsn;
+    ib_tab[i] = (int16_t)nsn;
   }
   if (tid == 0) dmax_s = 0;
   __syncthreads();
-  for (int i = tid; i < g.nli * NB; i += BP_THREADS) {
-    const int li = i / NB, b = i - li * NB;
+  for (int li = tid; li < g.nli; li += BP_THREADS) {
     float2 z = g.li_z[li];
     short2 sn = g.li_sn[li];
     int r0 = (int)floorf(z.x) >> 1, d = (int)z.y >> 1;
-    float sf = lor_sfac(z.y, g.vox_z, atn);
-    int k = (bp_row(d) + 2 + r0) * NB + b;
-    va[k] = in[sn.x * nbt + b] * sf;
-    if (d > 0) vb[k] = in[sn.y * nbt + b] * sf;
-    if (b == 0) atomicMax(&dmax_s, d);
+    int k = bp_row(d) + 2 + r0;
+    ia_tab[k] = sn.x;
+    if (d > 0) ib_tab[k] = sn.y;
+    atomicMax(&dmax_s, d);
+  }
+  __syncthreads();
+  for (int i = tid; i < BP_TABN; i += BP_THREADS) {
+    const int prev = i > 0 ? ia_tab[i - 1] : nsn, next = i + 1 < BP_TABN ? ib_tab[i + 1] : nsn;
+    pa[i] = (prev << 16) | (ia_tab[i] & 0xffff);
+    pb[i] = (next << 16) | (ib_tab[i] & 0xffff);
   }
   __syncthreads();
   const int dmax = dmax_s;
-  // per-lane constants: d = lane, slope 2d/atn of zeta along t
   const double m_lane = 2.0 * lane / atn;
   const double inv_lane = lane > 0 ? atn / (2.0 * lane) : 0.0;
+  const float sf_lane = lor_sfac(2.f * lane, g.vox_z, atn);
   const int q = lane;
+  // direct planes: plane 2q <- L * va0[q]
+  float X0[NB];
+  ldv<NB>(X0, srow + (ia_tab[bp_row(0) + 2 + q] & 0xffff) * NB);
+  float *wb = wbuf[wave];
 
-  for (int c = 0; c < nseg; c += CH) {
-    __syncthreads();
-    if (tid < CH) {
-      SegRec z;
-      z.s = 0.0;
-      z.len = 0.f;
-      z.colz = 0;
-      sbuf[tid] = (c + tid < nseg) ? g.seg[beg + c + tid] : z;
-    }
-    __syncthreads();
-    for (int j = wave; j < CH; j += BP_THREADS / 64) {
-      const SegRec sg = sbuf[j];
-      float P1 = 0.f, P2 = 0.f, P3 = 0.f;
-      int W0 = 0;
+  for (int g0 = wave * SPW; g0 < nseg; g0 += NWAVE * SPW) {
+    // lane d: base plane and pieces (times the 3D factor) of each segment
+    float P1[SPW], P2[SPW], P3[SPW], L[SPW];
+    int W0[SPW], C[SPW];
+#pragma unroll
+    for (int s = 0; s < SPW; s++) {
+      const int k = g0 + s;
+      SegRec sg;
+      if (k < nseg) {
+        sg = g.seg[beg + k];
+      } else {
+        sg.s = 0.0;
+        sg.len = 0.f;
+        sg.colz = 0;
+      }
+      L[s] = sg.len;
+      C[s] = sg.colz;
+      P1[s] = P2[s] = P3[s] = 0.f;
+      W0[s] = 0;
       if (lane >= 1 && lane <= dmax) {
         double w0 = floor(0.5 + m_lane * sg.s);
         double tb = (w0 + 0.5) * inv_lane;
         float q1 = (float)(tb - sg.s), q2 = (float)(tb + inv_lane - sg.s);
         float p1 = __builtin_amdgcn_fmed3f(q1, 0.f, sg.len);
         float c2 = __builtin_amdgcn_fmed3f(q2, 0.f, sg.len);
-        P1 = p1;
-        P2 = c2 - p1;
-        P3 = sg.len - c2;
-        W0 = (int)w0;
+        P1[s] = p1 * sf_lane;
+        P2[s] = (c2 - p1) * sf_lane;
+        P3[s] = (sg.len - c2) * sf_lane;
+        W0[s] = (int)w0;
       }
-      float acc0[NB], acc1[NB], X[NB], Y[NB], Xb[NB], Yb[NB];
-      ldv<NB>(X, va + (bp_row(0) + 2 + q) * NB);  // direct planes: plane 2q <- L va0[q]
+    }
+    float acc0[SPW][NB], acc1[SPW][NB];
+#pragma unroll
+    for (int s = 0; s < SPW; s++)
 #pragma unroll
       for (int b = 0; b < NB; b++) {
-        acc0[b] = sg.len * X[b];
-        acc1[b] = 0.f;
+        acc0[s][b] = L[s] * X0[b];
+        acc1[s][b] = 0.f;
       }
-      for (int d = 1; d <= dmax; d++) {
-        const float p1 = rdlane(P1, d), p2 = rdlane(P2, d), p3 = rdlane(P3, d);
-        const int w0 = __builtin_amdgcn_readlane(W0, d);
-        const int h = w0 >> 1, e = (2 * d - w0) >> 1;
-        const int rb = bp_row(d) + 2;
-        // clamped indices always land on zero pads (r0 < 0 or r0 > 63 - d)
-        const int ia = rb + clampi(q - h, -1, 65 - d);
-        const int ib = rb + clampi(q - e, -2, 64 - d);
-        ldv<NB>(X, va + ia * NB);
-        ldv<NB>(Y, va + (ia - 1) * NB);
-        ldv<NB>(Xb, vb + ib * NB);
-        ldv<NB>(Yb, vb + (ib + 1) * NB);
+    bool any3 = false;
+#pragma unroll
+    for (int s = 0; s < SPW; s++) any3 |= __any(P3[s] != 0.f);
+
+    for (int d = 1; d <= dmax; d++) {
+      const int rb = bp_row(d) + 2;
+      float S1[NB], S3[NB], Te[NB], To[NB];
+      int wprev = -1000;
+#pragma unroll
+      for (int s = 0; s < SPW; s++) {
+        const int w0 = __builtin_amdgcn_readlane(W0[s], d);
+        if (w0 != wprev) {  // wave-uniform: new base plane -> reload the LOR values
+          wprev = w0;
+          const int h = w0 >> 1, e = (2 * d - w0) >> 1;
+          const int ka = rb + clampi(q - h, -1, 65 - d);
+          const int kb = rb + clampi(q - e, -2, 64 - d);
+          const int ta = pa[ka], tbb = pb[kb];
+          float X[NB], Y[NB], Xb[NB], Yb[NB];
+          ldv<NB>(X, srow + (ta & 0xffff) * NB);
+          ldv<NB>(Y, srow + (ta >> 16) * NB);
+          ldv<NB>(Xb, srow + (tbb & 0xffff) * NB);
+          ldv<NB>(Yb, srow + (tbb >> 16) * NB);
+#pragma unroll
+          for (int b = 0; b < NB; b++) {
+            S1[b] = X[b] + Xb[b];
+            S3[b] = Y[b] + Yb[b];
+            Te[b] = X[b] + Yb[b];
+            To[b] = Y[b] + Xb[b];
+          }
+        }
+        const float p1 = rdlane(P1[s], d), p2 = rdlane(P2[s], d);
         if (w0 & 1) {
 #pragma unroll
           for (int b = 0; b < NB; b++) {
-            acc0[b] = fmaf(p2, Y[b] + Xb[b], acc0[b]);
-            acc1[b] = fmaf(p1, X[b] + Xb[b], acc1[b]);
-            acc1[b] = fmaf(p3, Y[b] + Yb[b], acc1[b]);
+            acc0[s][b] = fmaf(p2, To[b], acc0[s][b]);
+            acc1[s][b] = fmaf(p1, S1[b], acc1[s][b]);
           }
         } else {
 #pragma unroll
           for (int b = 0; b < NB; b++) {
-            acc0[b] = fmaf(p1, X[b] + Xb[b], acc0[b]);
-            acc0[b] = fmaf(p3, Y[b] + Yb[b], acc0[b]);
-            acc1[b] = fmaf(p2, X[b] + Yb[b], acc1[b]);
+            acc0[s][b] = fmaf(p1, S1[b], acc0[s][b]);
+            acc1[s][b] = fmaf(p2, Te[b], acc1[s][b]);
+          }
+        }
+        if (any3) {
+          const float p3 = rdlane(P3[s], d);
+          if (w0 & 1) {
+#pragma unroll
+            for (int b = 0; b < NB; b++) acc1[s][b] = fmaf(p3, S3[b], acc1[s][b]);
+          } else {
+#pragma unroll
+            for (int b = 0; b < NB; b++) acc0[s][b] = fmaf(p3, S3[b], acc0[s][b]);
           }
         }
       }
+    }
+    // flush the wave's segments through its LDS buffer (wave-local, no barrier)
+#pragma unroll
+    for (int s = 0; s < SPW; s++) {
+      if (g0 + s >= nseg) break;
 #pragma unroll
       for (int b = 0; b < NB; b++) {
-        zs[(j * ZSTRIDE + 2 * q) * NB + b] = acc0[b];
-        zs[(j * ZSTRIDE + 2 * q + 1) * NB + b] = acc1[b];
+        wb[(2 * q) * NB + b] = acc0[s][b];
+        wb[(2 * q + 1) * NB + b] = acc1[s][b];
       }
-    }
-    __syncthreads();
-    // flush the chunk: one coalesced global atomic per (segment, plane, replicate)
-    const int nj = min(CH, nseg - c);
-    for (int idx = tid; idx < nj * ZSTRIDE * NB; idx += BP_THREADS) {
-      const int j = idx / (ZSTRIDE * NB), rem = idx - j * (ZSTRIDE * NB);
-      const int z = rem / NB, b = rem - z * NB;
-      const float v = zs[idx];
-      if (z < NZ && v != 0.f)
-        unsafeAtomicAdd(img + ((int64_t)sbuf[j].colz + z) * nbt + b0 + b, v);
+      __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the wave's LDS writes landed
+      __builtin_amdgcn_wave_barrier();
+      const int colz = __builtin_amdgcn_readfirstlane(C[s]);
+      for (int i = lane; i < NZ * NB; i += 64) {
+        const int z = i / NB, b = i - z * NB;
+        const float v = wb[i];
+        if (v != 0.f) unsafeAtomicAdd(img + ((int64_t)colz + z) * nbt + b0 + b, v);
+      }
+      __builtin_amdgcn_wave_barrier();
     }
   }
 }
