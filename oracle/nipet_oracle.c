@@ -361,3 +361,134 @@ int orc_max_threads(void) {
   return 1;
 #endif
 }
+
+/* ---------------------------------------------------------------------------
+ * Finite-difference checker of the Poisson log-likelihood gradient.
+ *
+ * L(x, mu) = sum over rays[] and bins b of  ybar_b - y_b log ybar_b  (the
+ * y_b log term only where y_b > 0), ybar_b = a_b(mu) n_b p_b(x) + rs_b, with p
+ * the emission projection of x and a the attenuation projection of mu (both
+ * as in fprj_impl, FP64). grad[i] = (L(v + h) - L(v - h)) / (2 h) for voxel
+ * vox[i] of x (wrt = 0) or of mu (wrt = 1); the other voxels and image are
+ * held. Only the rays whose transaxial list holds the voxel's column are
+ * re-evaluated: the terms of the other rays cancel exactly in the difference.
+ */
+
+/* projection of one ray into out[nsn] (zeroed here) with img[ov] replaced by
+ * ovv (ov < 0: no replacement) */
+static void ray_proj(const double *img, int64_t ov, double ovv, const ray_t *r, int nseg,
+                     const int32_t *col, const double *ts, const double *len,
+                     const float *li2rng, const int16_t *li2sn, const int8_t *li2nos, int nli,
+                     int nsn, int att, double *out) {
+  memset(out, 0, nsn * sizeof(double));
+  for (int li = 0; li < nli; li++) {
+    lor_t L;
+    lor_setup(li2rng, li, r->atn, &L);
+    double acc = 0, accb = 0;
+    int64_t np = 0;
+#define IMGV(k) ((int64_t)(k) == ov ? ovv : img[k])
+#define ACC_A(w, c, wt) (acc += (wt) * IMGV((size_t)(c) * NZ + (w)),                          \
+                         accb += (wt) * IMGV((size_t)(c) * NZ + (L.wsum - (w))))
+    WALK_LOR(L, nseg, col, ts, len, ACC_A, np);
+#undef ACC_A
+#undef IMGV
+    (void)np;
+    double fa = att ? exp(-acc) / (double)li2nos[li] : acc;
+    out[li2sn[2 * li]] += fa;
+    if (!L.direct) out[li2sn[2 * li + 1]] += att ? exp(-accb) / (double)li2nos[li] : accb;
+  }
+}
+
+static double ray_nll(const double *p, const double *a, const double *y, const double *n,
+                      const double *rs, int nsn) {
+  double s = 0;
+  for (int b = 0; b < nsn; b++) {
+    double yb = a[b] * n[b] * p[b] + rs[b];
+    s += yb;
+    if (y[b] > 0) s -= y[b] * log(yb);
+  }
+  return s;
+}
+
+int orc_nll_fd(const double *x, const double *mu, const int32_t *rays, int nray,
+               const double *y, const double *nrm, const double *rs, const float *crs,
+               const int16_t *s2c, const float *li2rng, const int16_t *li2sn,
+               const int8_t *li2nos, int nli, int nsn, const int64_t *vox, int nv, int wrt,
+               double h, double *grad) {
+  /* per ray: transaxial list, column mask, base projections */
+  int32_t **cols = (int32_t **)calloc(nray, sizeof(void *));
+  double **tss = (double **)calloc(nray, sizeof(void *));
+  double **lns = (double **)calloc(nray, sizeof(void *));
+  double *pbase = (double *)calloc((size_t)nray * nsn, sizeof(double));
+  double *abase = (double *)calloc((size_t)nray * nsn, sizeof(double));
+  uint8_t *cmask = (uint8_t *)calloc((size_t)nray * NX * NY, 1);
+  ray_t *rr = (ray_t *)calloc(nray, sizeof(ray_t));
+  int *nsg = (int *)calloc(nray, sizeof(int));
+  int err = !cols || !tss || !lns || !pbase || !abase || !cmask || !rr || !nsg;
+  for (int i = 0; i < nray && !err; i++) {
+    cols[i] = (int32_t *)malloc(MAXCROSS * sizeof(int32_t));
+    tss[i] = (double *)malloc(MAXCROSS * sizeof(double));
+    lns[i] = (double *)malloc(MAXCROSS * sizeof(double));
+    if (!cols[i] || !tss[i] || !lns[i]) {
+      err = 1;
+      break;
+    }
+    int aw = rays[i];
+    nsg[i] = trace_tx(crs, s2c[2 * aw], s2c[2 * aw + 1], &rr[i], cols[i], tss[i], lns[i]);
+    if (nsg[i] < 0) {
+      err = 1;
+      break;
+    }
+    for (int k = 0; k < nsg[i]; k++) cmask[(size_t)i * NX * NY + cols[i][k]] = 1;
+  }
+  if (!err) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < nray; i++) {
+      ray_proj(x, -1, 0.0, &rr[i], nsg[i], cols[i], tss[i], lns[i], li2rng, li2sn, li2nos, nli,
+               nsn, 0, pbase + (size_t)i * nsn);
+      ray_proj(mu, -1, 0.0, &rr[i], nsg[i], cols[i], tss[i], lns[i], li2rng, li2sn, li2nos, nli,
+               nsn, 1, abase + (size_t)i * nsn);
+    }
+#pragma omp parallel
+    {
+      double *q = (double *)malloc(nsn * sizeof(double));
+#pragma omp for schedule(dynamic, 1)
+      for (int k = 0; k < nv; k++) {
+        int64_t v = vox[k];
+        int c = (int)(v / NZ);
+        const double *img = wrt ? mu : x;
+        double d = 0;
+        for (int i = 0; i < nray; i++) {
+          if (!cmask[(size_t)i * NX * NY + c]) continue;
+          size_t o = (size_t)i * nsn;
+          double lp, lm;
+          ray_proj(img, v, img[v] + h, &rr[i], nsg[i], cols[i], tss[i], lns[i], li2rng, li2sn,
+                   li2nos, nli, nsn, wrt, q);
+          lp = wrt ? ray_nll(pbase + o, q, y + o, nrm + o, rs + o, nsn)
+                   : ray_nll(q, abase + o, y + o, nrm + o, rs + o, nsn);
+          ray_proj(img, v, img[v] - h, &rr[i], nsg[i], cols[i], tss[i], lns[i], li2rng, li2sn,
+                   li2nos, nli, nsn, wrt, q);
+          lm = wrt ? ray_nll(pbase + o, q, y + o, nrm + o, rs + o, nsn)
+                   : ray_nll(q, abase + o, y + o, nrm + o, rs + o, nsn);
+          d += lp - lm;
+        }
+        grad[k] = d / (2.0 * h);
+      }
+      free(q);
+    }
+  }
+  for (int i = 0; i < nray && cols; i++) {
+    free(cols[i]);
+    free(tss[i]);
+    free(lns[i]);
+  }
+  free(cols);
+  free(tss);
+  free(lns);
+  free(pbase);
+  free(abase);
+  free(cmask);
+  free(rr);
+  free(nsg);
+  return err ? -1 : 0;
+}

@@ -36,6 +36,9 @@ def lib():
         for f in (L.orc_tx_lists, L.orc_fprj, L.orc_bprj):
             f.restype = ct.c_int
         L.orc_max_threads.restype = ct.c_int
+        L.orc_nll_fd.argtypes = [P, P, P, ct.c_int, P, P, P, P, P, P, P, P, ct.c_int, ct.c_int,
+                                 P, ct.c_int, ct.c_int, ct.c_double, P]
+        L.orc_nll_fd.restype = ct.c_int
         L.orc_philox4x32.argtypes = [P, P, P]
         L.orc_poisson_cdf.argtypes = [ct.c_double, P, ct.c_int]
         L.orc_lm_hist.argtypes = [P, ct.c_int64, ct.c_int64, ct.c_int, ct.c_int, ct.c_int,
@@ -134,6 +137,29 @@ def osem(params, prompts, rho, sens, mask, subsets, img0, niter, eps=1e-6):
             b = np.where(np.abs(sv) < eps, 0.0, b / np.where(np.abs(sv) < eps, 1.0, sv))
             x = np.where(mask, x * b, 0.0)
     return x
+
+
+def nll_fd(params, x, mu, rays, y, nrm, rs, vox, wrt, h):
+    """FP64 central finite differences of the Poisson negative log-likelihood
+    sum_b ybar_b - y_b log ybar_b, ybar = a(mu) n p(x) + rs over the transaxial
+    bins ``rays`` (y, nrm, rs: (len(rays), nsinos)), with respect to voxels
+    ``vox`` (flat indices of the (320,320,127) image) of x (wrt = 0) or of mu
+    (wrt = 1), step h."""
+    crs, s2c, li2rng, li2sn, li2nos, nsn = _geom(params)
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    mu = np.ascontiguousarray(mu, dtype=np.float64).ravel()
+    rays = np.ascontiguousarray(rays, dtype=np.int32)
+    y, nrm, rs = (np.ascontiguousarray(a, dtype=np.float64) for a in (y, nrm, rs))
+    for a in (y, nrm, rs):
+        assert a.shape == (len(rays), nsn)
+    vox = np.ascontiguousarray(vox, dtype=np.int64)
+    assert vox.min() >= 0 and vox.max() < NVOX
+    g = np.zeros(len(vox))
+    rc = lib().orc_nll_fd(_p(x), _p(mu), _p(rays), len(rays), _p(y), _p(nrm), _p(rs), _p(crs),
+                          _p(s2c), _p(li2rng), _p(li2sn), _p(li2nos), len(li2rng), nsn, _p(vox),
+                          len(vox), int(wrt), float(h), _p(g))
+    assert rc == 0
+    return g
 
 
 def philox4x32(ctr, key):

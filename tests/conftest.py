@@ -55,3 +55,29 @@ def gpu_ok():
         return torch.cuda.is_available()
     except Exception:
         return False
+
+
+def nll_case(params, rays, seed=11, nvox=40):
+    """Poisson-likelihood test case over transaxial bins ``rays``: current
+    image x (phantom 3), mu-map, norm U(0.8, 1.2), randoms+scatter, prompts
+    y ~ Poisson(a n A x_true + rs) with x_true = phantom 1 (FP64 oracle
+    projections), and ``nvox`` voxels on the rays' columns (flat indices)."""
+    from oracle import oracle as O
+    from nipet_amd.prj.mmrsim import mumap_phantom
+    rng = np.random.default_rng(seed)
+    rays = np.asarray(rays, np.int32)
+    xt, x, mu = phantom(1), phantom(3), mumap_phantom()
+    a = O.fprj(params, mu, rays, att=True)
+    p = O.fprj(params, xt, rays)
+    # norm and rs are float32 values (what a float32 device model sees)
+    n = rng.uniform(0.8, 1.2, a.shape).astype(np.float32).astype(np.float64)
+    rs = (0.3 * (a * n * p).mean() * rng.uniform(0.5, 1.5, a.shape)).astype(np.float32)
+    rs = rs.astype(np.float64)
+    y = rng.poisson(a * n * p + rs).astype(np.float64)
+    off, col, _, _, _, _ = O.tx_lists(params, rays)
+    vox = np.zeros(0, np.int64)
+    while len(vox) < nvox:
+        k = rng.integers(0, off[-1], nvox)
+        vox = np.unique(np.concatenate([vox, col[k].astype(np.int64) * 127 +
+                                        rng.integers(2, 125, nvox)]))
+    return dict(rays=rays, x=x, mu=mu, n=n, rs=rs, y=y, vox=rng.permutation(vox)[:nvox])
