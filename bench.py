@@ -15,7 +15,15 @@ with a bone shell 0.15/cm; prompts = Poisson(scale (a n FP(phantom)) + randoms
 + scatter) at 3e8 expected counts; norm U(0.8, 1.2); randoms flat 12 % and
 scatter smooth 25 % of the trues.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Batched bootstrap (extra field "batched", skipped by --no-batch): a synthetic
+600 s list-mode stream of 6e8 words (1 ms tags, 85 % prompts drawn from
+a n FP(phantom), 15 % delayeds) is histogrammed once into 64 Poisson(1)
+bootstrap replicates and reconstructed 4x14 as one batch (one HIP graph
+launch, autotuned sub-batch widths); it is compared with plain
+reconstructions of single replicates (one histogram session with that
+replicate, osem_prepare, osem_run), whose images it must equal.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch]
 """
 import argparse
 import json
@@ -44,6 +52,9 @@ def parse():
     ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    ap.add_argument('--no-batch', action='store_true', help='skip the batched bootstrap leg')
+    ap.add_argument('--nrep', type=int, default=64, help='bootstrap replicates of the batch')
+    ap.add_argument('--lm-words', type=float, default=6e8, help='list-mode words (600 s)')
     ap.add_argument('--cpu-seconds', type=float, default=12.0,
                     help='approximate CPU work of the baseline sample')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='BYTES',
@@ -102,6 +113,61 @@ def copy_calibration(nbytes):
     torch.cuda.synchronize()
     print(json.dumps({'copy_bytes_read': 4 * n, 'copy_bytes_written': 4 * n, 'launches': 3}),
           flush=True)
+
+
+def batched_bootstrap(params, d, nrep, nwords, nplain=2):
+    """Batched vs plain bootstrap reconstructions (see the module docstring)."""
+    import torch
+
+    from nipet_amd.lm.lmsim import synth_lm
+    from nipet_amd.lm.mmrhist import hist_words
+    from nipet_amd.prj.batch import _remgaps_b, bootstrap_osem
+    from nipet_amd.prj.mmrprj import frwd_prj
+    from nipet_amd.prj.osem import osem_prepare, osem_run
+    dur = 600
+    per_ms = max(1, int(round(nwords / (dur * 1000))))
+    emis = frwd_prj(torch.from_numpy(d['img']).cuda(), params, dev_out=True)
+    log(f'synthesising {dur} s list-mode stream ({per_ms} events/ms)')
+    words = synth_lm(params, dur, per_ms, weights=d['attn'] * d['norm'] * emis, seed=5)
+    del emis
+    rs = (d['randoms'] + d['scatter']) * 1e-3
+    seed = 2024
+    torch.cuda.synchronize()
+    log(f'{words.numel()} words; batched bootstrap of {nrep} replicates')
+    t0 = time.perf_counter()
+    imgs, info = bootstrap_osem(words, params, d['attn'], d['norm'], rs, nrep=nrep, seed=seed,
+                                itr=NITER)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    log(f'batched: {t_batch:.2f} s for {nrep}; widths {info["tune"]["fprj"]}/'
+        f'{info["tune"]["bprj"]}')
+    t_plain, rels = [], []
+    for r in range(nplain):
+        rep = (r * 37) % nrep
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        h = hist_words(words, params, btp=2, seed=seed, nrep=1, rep0=rep)
+        p = _remgaps_b(h, params, 1, 0, 1)[..., 0].contiguous()
+        del h
+        st = osem_prepare(params, p, d['attn'], d['norm'], rs)
+        img = osem_run(st, NITER, check_counts=False)
+        torch.cuda.synchronize()
+        t_plain.append(time.perf_counter() - t0)
+        rels.append(float(torch.linalg.vector_norm((imgs[..., rep] - img).double()) /
+                          torch.linalg.vector_norm(img.double())))
+        del st, img, p
+    tp = sum(t_plain) / len(t_plain)
+    log(f'plain: {tp:.3f} s per replicate; max rel L2 {max(rels):.3g}')
+    return {'workload': f'{nrep} Poisson(1) bootstrap replicates of a {dur} s list-mode stream '
+                        f'of {words.numel()} words, OSEM {NITER}x{NSUB} each, span-11',
+            'recons_per_s': round(nrep / t_batch, 4),
+            'plain_recons_per_s': round(1.0 / tp, 4),
+            'speedup_vs_plain': round((nrep / t_batch) * tp, 3),
+            'batch_s': round(t_batch, 3), 'plain_s_per_recon': round(tp, 4),
+            'plain_checked': nplain, 'max_rel_l2_vs_plain': max(rels),
+            'widths': {'fprj': info['tune']['fprj'], 'bprj': info['tune']['bprj']},
+            'timing': 'wall clock incl. histogramming, gap removal, corrections, '
+                      'sensitivity, autotune, graph capture'}
 
 
 def main():
@@ -188,6 +254,8 @@ def main():
         'roofline': roofline,
     }
     log('byte model done')
+    if not args.no_batch:
+        out['batched'] = batched_bootstrap(params, d, args.nrep, args.lm_words)
     if not args.no_cpu:
         log('CPU baseline')
         out['cpu_baseline'] = cpu_baseline(params, args.cpu_seconds)

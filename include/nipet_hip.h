@@ -96,6 +96,15 @@ int nipet_bprj(const NipetGeom *g, float *img, const float *sino, const int32_t 
 int nipet_remgaps(float *out, const void *in, int in_dtype, const int32_t *aw2ali, int naw,
                   int nsn, int nbinang, void *stream);
 
+/* Gap removal of a replicate-innermost histogram: in[(s * nbinang + ali) * nrep
+ * + r] (uint32 counts, the layout nipet_lm_chunk writes for nrep replicates)
+ * -> out[(aw * nsn + s) * nb + b] = min(in[.. r0 + b], 65535) as uint16, the
+ * batched prompts of nipet_osem_subset_b. The reference removes gaps per
+ * frame (mmraux.py:1146-1160, src/scanner_0.cu:94-155) and hands petprj.osem
+ * uint16 prompts (prj/src/prj_module.cu:649-651). */
+int nipet_remgaps_u16b(uint16_t *out, const uint32_t *in, const int32_t *aw2ali, int naw,
+                       int nsn, int nbinang, int nrep, int r0, int nb, void *stream);
+
 /* Gap insertion: device sinogram in[naw][nsn] -> Siemens out[nsn][nbinang]
  * (gaps zero), out[s][aw2ali[aw]] = in[aw][s]. Replaces mmr_auxe.pgaps /
  * put_gaps (src/scanner_0.cu:158-241) and the transpose of mmraux.py:1141. */
@@ -154,6 +163,8 @@ typedef struct {
   double btprt;          /* Poisson mean of the bootstrap weights (Cnt['BTPRT']) */
   uint64_t seed;         /* Philox4x32-10 key */
   int nrep;              /* replicates 1..64; nrep > 1 stores [bin][nrep] */
+  int rep0;              /* replicate index of the first stored replicate (Philox counter
+                            word 2 = rep0 + r), so sessions can split a replicate set */
   const int16_t *sn1_sn11;  /* host [4084] (axLUT['sn1_sn11']) */
   const int16_t *sn1_ssrb;  /* host [4084] (axLUT['sn1_ssrb']) */
   const int16_t *sn1_rno;   /* host [4084][2] (axLUT['sn1_rno']) */
@@ -199,6 +210,13 @@ int nipet_philox4x32(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
 #define NIPET_PROF_FILL 3
 #define NIPET_PROF_HIST 4
 int nipet_prof_enable(int on);
+/* Largest replicate sub-batch (4, 2 or 1) that one FP (kernel =
+ * NIPET_PROF_FPRJ) or BP (NIPET_PROF_BPRJ) launch covers in the batched entry
+ * points; batches are split greedily into sub-batches of at most that width.
+ * Process-wide; set by the autotuner (no counterpart in the reference, whose
+ * projectors take one image). */
+int nipet_tune_set(int kernel, int width);
+int nipet_tune_get(int kernel, int *width);
 int nipet_prof_reset(void);
 int nipet_prof_read(int kernel, double *total_ms, int64_t *count);
 

@@ -26,7 +26,7 @@ class NipetLmConfig(ct.Structure):
     _fields_ = [
         ('toff', ct.c_int), ('nitag', ct.c_int), ('tstart', ct.c_int), ('tstop', ct.c_int),
         ('span', ct.c_int), ('btp', ct.c_int), ('btprt', ct.c_double), ('seed', ct.c_uint64),
-        ('nrep', ct.c_int), ('sn1_sn11', ct.c_void_p), ('sn1_ssrb', ct.c_void_p),
+        ('nrep', ct.c_int), ('rep0', ct.c_int), ('sn1_sn11', ct.c_void_p), ('sn1_ssrb', ct.c_void_p),
         ('sn1_rno', ct.c_void_p), ('s2cF', ct.c_void_p)]
 
 
@@ -64,6 +64,9 @@ def lib():
             'nipet_osem_corr': ([P, P, P, P, P, P, I64, P], I),
             'nipet_remgaps': ([P, P, I, P, I, I, I, P], I),
             'nipet_putgaps': ([P, P, P, I, I, I, P], I),
+            'nipet_remgaps_u16b': ([P, P, P, I, I, I, I, I, I, P], I),
+            'nipet_tune_set': ([I, I], I),
+            'nipet_tune_get': ([I, P], I),
             'nipet_osem_subset': ([P, P, P, P, P, P, P, I, P, P, P], I),
             'nipet_fprj_b': ([P, P, P, P, I, I, I, P], I),
             'nipet_bprj_b': ([P, P, P, P, I, I, P], I),

@@ -13,6 +13,7 @@ constexpr int NY = 320;   // device image y (v)
 constexpr int NZ = 127;   // device image z (w), fastest
 constexpr int64_t NVOX = (int64_t)NX * NY * NZ;
 constexpr int MAXNSN = 4096;  // largest sinogram count handled (span-1: 4084)
+constexpr int LI_MAX = 2112;  // largest Michelogram (ring pair) count
 constexpr float EPS = 1e-6f;  // OSEM division guard (prj/src/recon.cu:12)
 
 // One transaxial segment: start t (cm from the first crystal, double so that
@@ -38,6 +39,12 @@ struct Geom {
   float2 *li_z = nullptr;    // [nli] (zeta_a at t=0, zeta_b - zeta_a) in z-voxel units
   short2 *li_sn = nullptr;   // [nli] output sinogram of LOR a and of its mirror b
   float *li_nosi = nullptr;  // [nli] 1 / (span-1 LORs per output sinogram)
+  // ring-difference view of the Michelogram: element li joins ring r0 (LOR a
+  // starts there) and ring r0 + d
+  int dmax = 0;                // largest ring difference d
+  int16_t *dr_li = nullptr;    // [64 * 64] li of (d, r0) at d * 64 + r0, -1 if absent
+  int32_t *sn_off = nullptr;   // [nsn + 1] CSR of the span-1 LORs of each output sinogram:
+  int16_t *sn_mem = nullptr;   // [2 nli] 2 li (LOR a) or 2 li + 1 (mirror b)
 };
 
 void set_error(const char *fmt, ...);

@@ -280,6 +280,9 @@ static void geom_free(Geom *g) {
   hipFree(g->li_z);
   hipFree(g->li_sn);
   hipFree(g->li_nosi);
+  hipFree(g->dr_li);
+  hipFree(g->sn_off);
+  hipFree(g->sn_mem);
 }
 
 static int geom_build(const NipetGeomDesc *d, NipetGeom *g) {
@@ -322,6 +325,55 @@ static int geom_build(const NipetGeomDesc *d, NipetGeom *g) {
     }
   }
   g->ndirect = ndirect;
+
+  // ring-difference tables: ring centres sit at zeta = 2 r + 0.5 (ring pitch =
+  // two z-voxels), so LOR a of li starts on ring r0 = (zeta_a - 0.5) / 2 and
+  // spans d = (zeta_b - zeta_a) / 2 rings
+  std::vector<int16_t> drli(64 * 64, -1);
+  std::vector<int32_t> snoff(d->nsn + 1, 0);
+  std::vector<int16_t> snmem;
+  if (d->nli > LI_MAX) {
+    set_error("more than %d Michelogram elements", LI_MAX);
+    return NIPET_EINVAL;
+  }
+  g->dmax = 0;
+  for (int li = 0; li < d->nli; li++) {
+    const float2 z = liz[li];
+    const int r0 = (int)floorf(z.x) >> 1, dd = (int)z.y >> 1;
+    if (z.x != 2.f * r0 + 0.5f || z.y != 2.f * dd || r0 < 0 || r0 + dd > 63) {
+      set_error("li2rng[%d]: rings must sit at z-voxel 2r + 0.5 of a 64-ring scanner", li);
+      return NIPET_EINVAL;
+    }
+    if (drli[dd * 64 + r0] >= 0) {
+      set_error("li2rng[%d]: ring pair listed twice", li);
+      return NIPET_EINVAL;
+    }
+    drli[dd * 64 + r0] = (int16_t)li;
+    g->dmax = dd > g->dmax ? dd : g->dmax;
+  }
+  for (int li = 0; li < d->nli; li++) {
+    snoff[lisn[li].x + 1]++;
+    if (liz[li].y > 0.f) snoff[lisn[li].y + 1]++;
+  }
+  for (int s = 0; s < d->nsn; s++) snoff[s + 1] += snoff[s];
+  snmem.assign(snoff[d->nsn] > 0 ? snoff[d->nsn] : 1, 0);
+  {
+    std::vector<int32_t> fill(snoff.begin(), snoff.end() - 1);
+    for (int li = 0; li < d->nli; li++) {
+      snmem[fill[lisn[li].x]++] = (int16_t)(2 * li);
+      if (liz[li].y > 0.f) snmem[fill[lisn[li].y]++] = (int16_t)(2 * li + 1);
+    }
+  }
+  NIPET_HIP(hipMalloc(&g->dr_li, drli.size() * sizeof(int16_t)));
+  NIPET_HIP(hipMalloc(&g->sn_off, snoff.size() * sizeof(int32_t)));
+  NIPET_HIP(hipMalloc(&g->sn_mem, snmem.size() * sizeof(int16_t)));
+  NIPET_HIP(hipMemcpy(g->dr_li, drli.data(), drli.size() * sizeof(int16_t),
+                      hipMemcpyHostToDevice));
+  NIPET_HIP(hipMemcpy(g->sn_off, snoff.data(), snoff.size() * sizeof(int32_t),
+                      hipMemcpyHostToDevice));
+  NIPET_HIP(hipMemcpy(g->sn_mem, snmem.data(), snmem.size() * sizeof(int16_t),
+                      hipMemcpyHostToDevice));
+
   NIPET_HIP(hipMalloc(&g->li_z, d->nli * sizeof(float2)));
   NIPET_HIP(hipMalloc(&g->li_sn, d->nli * sizeof(short2)));
   NIPET_HIP(hipMalloc(&g->li_nosi, d->nli * sizeof(float)));
@@ -375,6 +427,19 @@ static int geom_build(const NipetGeomDesc *d, NipetGeom *g) {
                           nullptr, g->seg, nullptr, g->lmax, d_err);
   NIPET_HIP(hipGetLastError());
   NIPET_HIP(hipDeviceSynchronize());
+  // the projectors split a segment into at most three z pieces: no segment may
+  // span two z-boundary spacings atn / (2 dmax) of the steepest LOR
+  {
+    std::vector<float> lm(d->naw);
+    std::vector<double> at(d->naw);
+    NIPET_HIP(hipMemcpy(lm.data(), g->lmax, d->naw * sizeof(float), hipMemcpyDeviceToHost));
+    NIPET_HIP(hipMemcpy(at.data(), g->atn, d->naw * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < d->naw; i++)
+      if ((double)lm[i] * 2.0 * g->dmax >= 1.99 * at[i]) {
+        set_error("transaxial bin %d: segments cross more than two z-planes", i);
+        return NIPET_EINVAL;
+      }
+  }
   hipFree(d_crs);
   hipFree(d_s2c);
   hipFree(d_cnt);

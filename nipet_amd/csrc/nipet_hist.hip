@@ -58,7 +58,7 @@ __host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
 struct LmDev {
   const int32_t *words;
   int64_t n, base;
-  int toff, nitag, tstart, tstop, span, btp, nrep;
+  int toff, nitag, tstart, tstop, span, btp, nrep, rep0;
   uint32_t k0, k1;
   int64_t nbins;
   const int32_t *lut;   // [NSN1] (sn1_sn11 << 16) | sn1_ssrb
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
       uint32_t N = 1;
       if (p.btp == 2) {
         uint64_t g = (uint64_t)(p.base + gi);
-        uint32_t cc[4] = {(uint32_t)g, (uint32_t)(g >> 32), 0u, 0u};
+        uint32_t cc[4] = {(uint32_t)g, (uint32_t)(g >> 32), (uint32_t)p.rep0, 0u};
         philox4x32_10(cc, p.k0, p.k1);
         N = poisson_n(cc[0], cdf);
       }
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
         if (lane < nr) {
           N = 1;
           if (p.btp == 2) {
-            uint32_t cc[4] = {(uint32_t)g, (uint32_t)(g >> 32), (uint32_t)lane, 0u};
+            uint32_t cc[4] = {(uint32_t)g, (uint32_t)(g >> 32), (uint32_t)(p.rep0 + lane), 0u};
             philox4x32_10(cc, p.k0, p.k1);
             N = poisson_n(cc[0], cdf);
           }
@@ -508,7 +508,8 @@ int nipet_lm_create(const NipetLmConfig *c, NipetLm **out) {
   }
   if (c->nitag <= 0 || c->tstart < 0 || c->tstop < c->tstart ||
       (c->span != 11 && c->span != 1 && c->span != 0) || (c->btp != 0 && c->btp != 2) ||
-      c->nrep < 1 || c->nrep > LM_MAXREP || !(c->btprt > 0.0 && c->btprt < 20.0)) {
+      c->nrep < 1 || c->nrep > LM_MAXREP || c->rep0 < 0 || c->rep0 > (1 << 30) ||
+      !(c->btprt > 0.0 && c->btprt < 20.0)) {
     set_error("nipet_lm_create: bad configuration (nitag=%d span=%d btp=%d nrep=%d)", c->nitag,
               c->span, c->btp, c->nrep);
     return NIPET_EINVAL;
@@ -590,6 +591,7 @@ int nipet_lm_chunk(NipetLm *h, const int32_t *words, int64_t n, const NipetLmOut
   p.span = c.span;
   p.btp = c.btp;
   p.nrep = c.nrep;
+  p.rep0 = c.rep0;
   p.k0 = (uint32_t)c.seed;
   p.k1 = (uint32_t)(c.seed >> 32);
   p.nbins = 0;

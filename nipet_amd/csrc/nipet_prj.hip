@@ -2,43 +2,32 @@
 //
 // Work decomposition (one workgroup = one transaxial bin of the subset):
 //  - the bin's transaxial segment list (FP64-built, device resident) is
-//    walked in chunks of CHUNK segments; for each chunk the z-columns
-//    img[col][0:127] of its voxels are staged in LDS (127 contiguous floats
-//    each: coalesced 508-byte reads) -- the FP gathers then never touch HBM;
-//  - each lane owns Michelogram elements li (LOR a = ring r0 -> r1 and its
-//    mirror b = r1 -> r0, which share their z-boundary parameters, reference
-//    prj/src/prjf.cu:137-190), li = lane + 704*r for r < 3 (2074 elements);
-//  - per segment a LOR needs its z-voxel w and the t of the next z-plane
-//    boundary tb; a segment contributes p1 = clamp(tb - s, 0, L) to w and the
-//    rest to w+1 (mirror: w_b and w_b-1), and a second boundary inside the
-//    segment (possible only when atn/dz < the longest segment of the bin) is
-//    handled by a wave-uniform slow path;
-//  - FP: span-1 -> span-11 reduction in LDS, then one coalesced store per
-//    sinogram bin (no global atomics; prj/src/prjf.cu:88-94 uses atomicAdd);
-//  - BP: a gather over ring differences into registers (see k_bprj), one
+//    walked in chunks; for each chunk the z-columns img[col][0:127] of its
+//    voxels are staged in LDS (127 contiguous floats each: coalesced 508-byte
+//    reads) -- the projections then never gather from HBM;
+//  - all LORs of one ring difference d share their z-boundary positions along
+//    the bin (ring centres at z-voxel 2r + 0.5), so the pieces of a segment
+//    are computed once per (segment, d) and shared by the up to 64 LORs of d
+//    (reference prj/src/prjf.cu:137-190 tracks them per LOR);
+//  - FP (k_fprj): lanes own z-plane pairs, LOR sums move between lanes with
+//    DPP shifts; span-1 -> span-11 sums gathered in a fixed order, one
+//    coalesced store per sinogram bin (prj/src/prjf.cu:88-94 uses atomicAdd);
+//  - BP (k_bprj): a gather over ring differences into registers, one
 //    coalesced global float atomic per (segment, plane) instead of one per
 //    (LOR, piece) (prj/src/prjb.cu:70-82, 153-185).
+#include <atomic>
+#include <utility>
+
 #include "nipet_common.h"
 
 namespace nipet {
 
-constexpr int PRJ_THREADS = 704;  // FP: 11 waves of 64
-constexpr int LI_ROUNDS = 3;      // 3*704 = 2112 >= 2074 Michelogram elements
 constexpr int BP_THREADS = 512;   // BP: 8 waves
-constexpr int SUBSUM = 8;         // FP: float partial sums over 8 segments
+constexpr int SUBSUM = 8;         // float partial sums over 8 segments
 constexpr int ZSTRIDE = 128;      // LDS z-column (127 + zero pad)
-constexpr int ZGUARD = 8;         // zero guard z-slots on both sides of the stage
-constexpr double TFAR = 1e300;    // boundary "never" (direct planes)
-constexpr double SPAD = -1e30;    // start of a padding segment (contributes nothing)
 constexpr int MAXNB = 4;          // replicates per kernel instance (larger batches: sub-batches)
 constexpr int BP_MAXD = 63;       // largest ring difference of the BP tables
 
-template <int NB>
-constexpr int fp_chunk() { return NB >= 4 ? 16 : 32; }
-template <int NB>
-constexpr int bp_chunk() { return NB == 1 ? 32 : (NB == 2 ? 16 : 8); }
-template <int NB>
-constexpr int fp_bins() { return NB == 1 ? MAXNSN : 837 * NB; }
 // BP table row d holds r0 = -2 .. 65-d (two zero pads on both sides); row offset:
 __host__ __device__ constexpr int bp_row(int d) { return 68 * d - d * (d - 1) / 2; }
 constexpr int BP_TABN = bp_row(BP_MAXD + 1);
@@ -51,13 +40,17 @@ struct GeomView {
   const float2 *li_z;
   const short2 *li_sn;
   const float *li_nosi;
-  int naw, nli, ndirect, nsn;
+  const int16_t *dr_li;
+  const int32_t *sn_off;
+  const int16_t *sn_mem;
+  int naw, nli, ndirect, nsn, dmax;
   float vox_z;
 };
 
 static GeomView view(const Geom *g) {
-  return GeomView{g->off, g->seg, g->atn, g->lmax, g->li_z, g->li_sn, g->li_nosi,
-                  g->naw, g->nli, g->ndirect, g->nsn, g->vox_z};
+  return GeomView{g->off,   g->seg,    g->atn,    g->lmax, g->li_z,    g->li_sn, g->li_nosi,
+                  g->dr_li, g->sn_off, g->sn_mem, g->naw,  g->nli,     g->ndirect, g->nsn,
+                  g->dmax,  g->vox_z};
 }
 
 // A sub-batch of NB replicates starting at replicate b0 of a batch of nbt:
@@ -86,119 +79,15 @@ __device__ inline void ldv(float (&r)[NB], const float *p) {
   }
 }
 
-// Per-lane state of one Michelogram element along a transaxial bin. The
-// boundary position tb is kept in double: positions reach 67 cm along the ray
-// where float32 resolves only 4e-6 cm, which would move weight between
-// adjacent z-planes at object edges.
-struct Lor {
-  double tb;   // t of the next z boundary of LOR a (same t for its mirror)
-  double inv;  // t between consecutive z boundaries (0: direct plane)
-  int ia, ib;  // z-voxel of LOR a and of its mirror b (ia + ib = 2 r0 + 2 r1)
-};
-
-__device__ inline void lor_init(Lor &L, float2 z, double atn, double s0) {
-  const double zeta0 = z.x, dz = z.y;  // exact (ring centres sit at 2r + 0.5)
-  const int wsum = (int)floorf(z.x) + (int)floorf(z.x + z.y);
-  if (z.y > 0.f) {
-    L.inv = atn / dz;
-    double w = floor(zeta0 + s0 / L.inv);
-    L.tb = (w + 1.0 - zeta0) * L.inv;
-    L.ia = (int)w;
-  } else {
-    L.inv = 0.0;
-    L.tb = TFAR;
-    L.ia = (int)floorf(z.x);
-  }
-  L.ib = wsum - L.ia;
+// Compile-time loop: f(std::integral_constant<int, i>) for i = 0 .. N-1, so
+// that register arrays are only ever indexed by constants.
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
 }
-
-// FP accumulation: pieces go into float partial sums over SUBSUM segments,
-// folded into the running totals (one long float sum of nearly equal segment
-// lengths would round in the same direction at every step).
-template <int NB>
-struct FpAcc {
-  float pa[NB], pb[NB], aa[NB], ab[NB];
-  __device__ void zero() {
-#pragma unroll
-    for (int b = 0; b < NB; b++) pa[b] = pb[b] = aa[b] = ab[b] = 0.f;
-  }
-  __device__ void piece(const float *col, int ia, int ib, float p, int k) {
-    float x[NB];
-    ldv<NB>(x, col + (ia + k) * NB);
-#pragma unroll
-    for (int b = 0; b < NB; b++) pa[b] = fmaf(p, x[b], pa[b]);
-    ldv<NB>(x, col + (ib - k) * NB);
-#pragma unroll
-    for (int b = 0; b < NB; b++) pb[b] = fmaf(p, x[b], pb[b]);
-  }
-  __device__ void direct(const float *col, int ia, float p) {
-    float x[NB];
-    ldv<NB>(x, col + ia * NB);
-#pragma unroll
-    for (int b = 0; b < NB; b++) pa[b] = fmaf(p, x[b], pa[b]);
-  }
-  __device__ void fold() {
-#pragma unroll
-    for (int b = 0; b < NB; b++) {
-      aa[b] += pa[b];
-      ab[b] += pb[b];
-      pa[b] = 0.f;
-      pb[b] = 0.f;
-    }
-  }
-};
-
-// One segment with at most one z boundary inside it.
-template <int NB>
-__device__ inline void step1(Lor &l, const SegRec &sg, const float *col, FpAcc<NB> &op) {
-  const float q = (float)(l.tb - sg.s);
-  const float p1 = __builtin_amdgcn_fmed3f(q, 0.f, sg.len);
-  op.piece(col, l.ia, l.ib, p1, 0);
-  op.piece(col, l.ia, l.ib, sg.len - p1, 1);
-  const bool cr = q <= sg.len;
-  l.ia += cr;
-  l.ib -= cr;
-  l.tb += cr ? l.inv : 0.0;
-}
-
-// One segment with up to two z boundaries inside it.
-template <int NB>
-__device__ inline void step2(Lor &l, const SegRec &sg, const float *col, FpAcc<NB> &op) {
-  const float q1 = (float)(l.tb - sg.s);
-  const float q2 = (float)(l.tb + l.inv - sg.s);
-  const float p1 = __builtin_amdgcn_fmed3f(q1, 0.f, sg.len);
-  const float c2 = __builtin_amdgcn_fmed3f(q2, 0.f, sg.len);
-  op.piece(col, l.ia, l.ib, p1, 0);
-  op.piece(col, l.ia, l.ib, c2 - p1, 1);
-  op.piece(col, l.ia, l.ib, sg.len - c2, 2);
-  const int n = (q1 <= sg.len) + (q2 <= sg.len);
-  l.ia += n;
-  l.ib -= n;
-  l.tb += (double)n * l.inv;
-}
-
-// Walk the CH staged segments for this lane's element of one round.
-template <int NB, int CH>
-__device__ inline void walk_chunk(Lor &l, FpAcc<NB> &op, const SegRec *sbuf, const float *zs,
-                                  bool direct_wave, bool two_wave) {
-  if (direct_wave) {
-    const float *col = zs + l.ia * NB;
-#pragma unroll
-    for (int j = 0; j < CH; j++) op.direct(col + j * ZSTRIDE * NB, 0, sbuf[j].len);
-    op.fold();
-  } else if (two_wave) {
-#pragma unroll 4
-    for (int j = 0; j < CH; j++) {
-      step2<NB>(l, sbuf[j], zs + j * ZSTRIDE * NB, op);
-      if ((j + 1) % SUBSUM == 0) op.fold();
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < CH; j++) {
-      step1<NB>(l, sbuf[j], zs + j * ZSTRIDE * NB, op);
-      if ((j + 1) % SUBSUM == 0) op.fold();
-    }
-  }
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
 __device__ inline float lor_sfac(float dz, float vox_z, double atn) {
@@ -207,114 +96,279 @@ __device__ inline float lor_sfac(float dz, float vox_z, double atn) {
   return sqrtf(1.f + r * r);
 }
 
-__device__ inline SegRec pad_seg() {
-  SegRec p;
-  p.s = SPAD;
-  p.len = 0.f;
-  p.colz = 0;
-  return p;
+__device__ inline float rdlane(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
 // ---------------------------------------------------------------------------
 // Forward projection. MODE 0: emission; 1: attenuation factors;
 // 2: OSEM ratio y / (FP + rho) with the 1e-6 guards of prj/src/recon.cu:252-283.
 // rho_nb: 0 = one rho shared by all replicates, else rho has the batch layout.
+//
+// Lane = Michelogram element li = lane + FP_THREADS r (r < FP_ROUNDS): LOR a
+// (ring r0 -> r1) and its mirror b (r1 -> r0) share their z-boundary
+// positions (reference prj/src/prjf.cu:137-190). Per segment [s, s + L] of
+// the staged chunk, LOR a is on plane ia and the next z boundary lies at
+// t = tb: it adds p1 = clamp(tb - s, 0, L) times plane ia and L - p1 times
+// plane ia + 1 (mirror: ib and ib - 1), and steps to the next plane when
+// tb <= s + L. tb is tracked in float relative to the chunk's first segment
+// start and recomputed exactly (FP64, from the plane index) at every chunk, so
+// positions along 67 cm rays keep 1e-6 cm resolution without FP64 in the
+// inner loop. Bins whose segments can hold two z boundaries (segment longer
+// than the boundary spacing) take a three-piece step. Pieces go into float
+// partial sums over SUBSUM segments folded into float totals (one long float
+// sum of nearly equal segment lengths would round in the same direction at
+// every step). Span-1 -> span-N: each output bin sums its LORs in a fixed
+// order from LDS (no atomics, bit-reproducible).
+constexpr int FP_THREADS = 704;  // 11 waves
+constexpr int FP_ROUNDS = 3;     // 3 * 704 = 2112 >= LI_MAX elements
+constexpr int FP_GUARD = 2;      // zero planes below 0 and above 127 (pieces reach ia + 2, ib - 2)
+constexpr int FP_PLANES = ZSTRIDE + 2 * FP_GUARD;
+
+// Staged chunk layout: value of (plane p, segment j, replicate b) at
+// ((p + FP_GUARD) * PS + j) * NB + b with PS = CH + 1: a lane's planes for
+// consecutive segments are adjacent (small immediate LDS offsets), and lanes
+// on different planes fall in different banks.
+// One chunk length for every batch width: the exact FP64 boundary rebase then
+// happens at the same segments, so a replicate's arithmetic (and result) is the
+// same whether it is projected alone or in a batch.
+template <int NB>
+constexpr int fp_chunk() { return 16; }
+template <int NB>
+constexpr int fp_ps() { return fp_chunk<NB>() + 1; }
+
+template <int NB>
+__device__ inline void fma_nb(float (&acc)[NB], float p, const float (&x)[NB]) {
+#pragma unroll
+  for (int b = 0; b < NB; b++) acc[b] = fmaf(p, x[b], acc[b]);
+}
+
+struct FpLor {
+  float tbr;   // next z boundary of LOR a relative to the chunk base (float)
+  float invf;  // boundary spacing (float)
+  int ao;      // staged-chunk offset of plane ia (LOR a), segment 0
+  int bo;      // staged-chunk offset of plane ib - 2 (mirror b: ib = 2 r0 + 2 r1 - ia)
+};
+
+template <int NB>
+struct FpSum {
+  float pa[NB], pb[NB], ta[NB], tb[NB];
+  __device__ void fold() {
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      ta[b] += pa[b];
+      tb[b] += pb[b];
+      pa[b] = 0.f;
+      pb[b] = 0.f;
+    }
+  }
+};
+
+// Walk the nch staged segments (SR: start - chunk base, length; padding
+// entries (-1e30, 0) contribute nothing and never cross). TWO: segments may
+// hold two z boundaries; DIRECT: direct planes only.
+template <int NB, bool DIRECT, bool TWO>
+__device__ inline void fp_walk(FpLor &l, FpSum<NB> &acc, const float2 *__restrict__ sr,
+                               const float *__restrict__ zs, int nch) {
+  constexpr int PS = fp_ps<NB>() * NB;  // plane stride (floats)
+  for (int j8 = 0; j8 < nch; j8 += SUBSUM) {
+    const float *za = zs + l.ao + j8 * NB;
+    const float *zb = zs + l.bo + j8 * NB;
+#pragma unroll(NB >= 4 ? 2 : SUBSUM)
+    for (int jj = 0; jj < SUBSUM; jj++) {
+      const float2 sl = sr[j8 + jj];
+      float xa0[NB], xa1[NB];
+      if constexpr (DIRECT) {
+        ldv<NB>(xa0, za + jj * NB);
+        fma_nb<NB>(acc.pa, sl.y, xa0);
+      } else if constexpr (!TWO) {
+        const float q = l.tbr - sl.x;
+        const float p1 = __builtin_amdgcn_fmed3f(q, 0.f, sl.y);
+        const float p2 = sl.y - p1;
+        float xb0[NB], xb1[NB];
+        ldv<NB>(xa0, za + jj * NB);
+        ldv<NB>(xa1, za + PS + jj * NB);
+        ldv<NB>(xb0, zb + 2 * PS + jj * NB);
+        ldv<NB>(xb1, zb + PS + jj * NB);
+        fma_nb<NB>(acc.pa, p1, xa0);
+        fma_nb<NB>(acc.pa, p2, xa1);
+        fma_nb<NB>(acc.pb, p1, xb0);
+        fma_nb<NB>(acc.pb, p2, xb1);
+        const bool cr = q <= sl.y;
+        const int step = cr ? PS : 0;
+        za += step;
+        zb -= step;
+        l.tbr += cr ? l.invf : 0.f;
+      } else {
+        const float q1 = l.tbr - sl.x, q2 = q1 + l.invf;
+        const float p1 = __builtin_amdgcn_fmed3f(q1, 0.f, sl.y);
+        const float c2 = __builtin_amdgcn_fmed3f(q2, 0.f, sl.y);
+        const float p2 = c2 - p1, p3 = sl.y - c2;
+        float xa2[NB], xb0[NB], xb1[NB], xb2[NB];
+        ldv<NB>(xa0, za + jj * NB);
+        ldv<NB>(xa1, za + PS + jj * NB);
+        ldv<NB>(xa2, za + 2 * PS + jj * NB);
+        ldv<NB>(xb0, zb + 2 * PS + jj * NB);
+        ldv<NB>(xb1, zb + PS + jj * NB);
+        ldv<NB>(xb2, zb + jj * NB);
+        fma_nb<NB>(acc.pa, p1, xa0);
+        fma_nb<NB>(acc.pa, p2, xa1);
+        fma_nb<NB>(acc.pa, p3, xa2);
+        fma_nb<NB>(acc.pb, p1, xb0);
+        fma_nb<NB>(acc.pb, p2, xb1);
+        fma_nb<NB>(acc.pb, p3, xb2);
+        const int n = (int)(q1 <= sl.y) + (int)(q2 <= sl.y);
+        za += n * PS;
+        zb -= n * PS;
+        l.tbr += (float)n * l.invf;
+      }
+    }
+    // za / zb moved by whole planes only (segments are immediate offsets)
+    l.ao = (int)(za - zs) - j8 * NB;
+    l.bo = (int)(zb - zs) - j8 * NB;
+    acc.fold();
+  }
+}
+
 template <int MODE, int NB>
-__global__ __launch_bounds__(PRJ_THREADS) void k_fprj(GeomView g, const float *__restrict__ img,
-                                                      float *__restrict__ sino,
-                                                      const int32_t *__restrict__ subs,
-                                                      const uint16_t *__restrict__ prompts,
-                                                      const float *__restrict__ rho, int rho_nb,
-                                                      Batch bt) {
-  constexpr int CH = fp_chunk<NB>();
-  __shared__ __align__(16) float zbuf[(ZGUARD + CH * ZSTRIDE + ZGUARD) * NB];
-  __shared__ SegRec sbuf[CH];
-  __shared__ float bins[fp_bins<NB>()];
-  float *zs = zbuf + ZGUARD * NB;
+__global__ __launch_bounds__(FP_THREADS) void k_fprj(GeomView g, const float *__restrict__ img,
+                                                     float *__restrict__ sino,
+                                                     const int32_t *__restrict__ subs,
+                                                     const uint16_t *__restrict__ prompts,
+                                                     const float *__restrict__ rho, int rho_nb,
+                                                     Batch bt) {
+  constexpr int CH = fp_chunk<NB>(), PS = fp_ps<NB>();
+  // staged chunk (plane-major, see fp_ps), then reused for the per-LOR values
+  constexpr int ZN = FP_PLANES * PS * NB, LN = 2 * LI_MAX;
+  __shared__ __align__(16) float lds[ZN > LN ? ZN : LN];
+  __shared__ float2 sr[CH];
   const int tid = threadIdx.x, wave = tid >> 6;
   const int row = blockIdx.x;
   const int aw = subs ? subs[row] : row;
-  const int nsn = g.nsn;
-  const int nbt = bt.nbt, b0 = bt.b0;
+  const int nsn = g.nsn, nbt = bt.nbt, b0 = bt.b0;
   float *out = sino + (size_t)row * nsn * nbt + b0;
   if (aw < 0 || aw >= g.naw) {  // invalid bin index: zero row
-    for (int i = tid; i < nsn * NB; i += PRJ_THREADS) out[(i / NB) * nbt + i % NB] = 0.f;
+    for (int i = tid; i < nsn * NB; i += FP_THREADS) out[(i / NB) * nbt + i % NB] = 0.f;
     return;
   }
   const int beg = g.off[aw], nseg = g.off[aw + 1] - beg;
   const double atn = g.atn[aw];
   const float lmax = g.lmax[aw];
-  for (int i = tid; i < nsn * NB; i += PRJ_THREADS) bins[i] = 0.f;
-  if (tid < ZGUARD * NB) {
-    zbuf[tid] = 0.f;
-    zbuf[(ZGUARD + CH * ZSTRIDE) * NB + tid] = 0.f;
+  float *zs = lds;
+  // guard planes (never written by the staging loop)
+  for (int i = tid; i < FP_GUARD * PS * NB; i += FP_THREADS) {
+    zs[i] = 0.f;
+    zs[(FP_GUARD + ZSTRIDE) * PS * NB + i] = 0.f;
   }
-  const double s0 = nseg > 0 ? g.seg[beg].s : 0.0;
 
-  Lor L[LI_ROUNDS];
-  FpAcc<NB> A[LI_ROUNDS];
-  bool two[LI_ROUNDS];
+  FpLor L[FP_ROUNDS];
+  FpSum<NB> A[FP_ROUNDS];
+  float z0[FP_ROUNDS];
+  double inv[FP_ROUNDS];
+  bool two[FP_ROUNDS], direct[FP_ROUNDS], live[FP_ROUNDS];
+  const double s0 = nseg > 0 ? g.seg[beg].s : 0.0;
 #pragma unroll
-  for (int r = 0; r < LI_ROUNDS; r++) {
-    int li = r * PRJ_THREADS + tid;
-    float2 z = li < g.nli ? g.li_z[li] : make_float2(0.5f, 0.f);
-    lor_init(L[r], z, atn, s0);
-    A[r].zero();
-    two[r] = __any(L[r].inv > 0.0 && L[r].inv < (double)lmax);
+  for (int r = 0; r < FP_ROUNDS; r++) {
+    const int li = r * FP_THREADS + tid;
+    const float2 z = li < g.nli ? g.li_z[li] : make_float2(0.5f, 0.f);
+    const int wsum = (int)floorf(z.x) + (int)floorf(z.x + z.y);
+    int ia;
+    z0[r] = z.x;
+    if (z.y > 0.f) {
+      inv[r] = atn / (double)z.y;
+      ia = (int)floor((double)z.x + s0 / inv[r]);
+    } else {
+      inv[r] = 0.0;
+      ia = (int)floorf(z.x);
+    }
+    L[r].ao = (ia + FP_GUARD) * PS * NB;
+    L[r].bo = (wsum - ia - 2 + FP_GUARD) * PS * NB;
+    L[r].invf = (float)inv[r];
+#pragma unroll
+    for (int b = 0; b < NB; b++) A[r].pa[b] = A[r].pb[b] = A[r].ta[b] = A[r].tb[b] = 0.f;
+    const int li0 = r * FP_THREADS + wave * 64;  // wave-uniform
+    live[r] = li0 < g.nli;
+    direct[r] = li0 + 64 <= g.ndirect;
+    two[r] = __any(inv[r] > 0.0 && inv[r] < (double)lmax);
   }
 
   for (int c = 0; c < nseg; c += CH) {
+    const int nch = nseg - c < CH ? nseg - c : CH;
+    const double sb = g.seg[beg + c].s;  // chunk base position
     __syncthreads();
-    if (tid < CH) sbuf[tid] = (c + tid < nseg) ? g.seg[beg + c + tid] : pad_seg();
-    for (int idx = tid; idx < CH * ZSTRIDE * NB; idx += PRJ_THREADS) {
+    if (tid < CH) {
+      const SegRec sg = g.seg[beg + (c + tid < nseg ? c + tid : c)];
+      sr[tid] = c + tid < nseg ? make_float2((float)(sg.s - sb), sg.len) : make_float2(-1e30f, 0.f);
+    }
+    for (int idx = tid; idx < CH * ZSTRIDE * NB; idx += FP_THREADS) {
       const int j = idx / (ZSTRIDE * NB), rem = idx - j * (ZSTRIDE * NB);
       const int z = rem / NB, b = rem - z * NB, k = c + j;
       float v = 0.f;
       if (k < nseg && z < NZ) v = img[((int64_t)g.seg[beg + k].colz + z) * nbt + b0 + b];
-      zs[idx] = v;
+      zs[((z + FP_GUARD) * PS + j) * NB + b] = v;
+    }
+    // exact next-boundary positions relative to the chunk base
+#pragma unroll
+    for (int r = 0; r < FP_ROUNDS; r++) {
+      const int ia = L[r].ao / (PS * NB) - FP_GUARD;
+      L[r].tbr = inv[r] > 0.0 ? (float)(((double)(ia + 1) - (double)z0[r]) * inv[r] - sb) : 3e38f;
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < LI_ROUNDS; r++) {
-      const int li0 = r * PRJ_THREADS + wave * 64;  // wave-uniform
-      if (li0 >= g.nli) continue;
-      walk_chunk<NB, CH>(L[r], A[r], sbuf, zs, li0 + 64 <= g.ndirect, two[r]);
+    for (int r = 0; r < FP_ROUNDS; r++) {
+      if (!live[r]) continue;
+      if (direct[r])
+        fp_walk<NB, true, false>(L[r], A[r], sr, zs, nch);
+      else if (two[r])
+        fp_walk<NB, false, true>(L[r], A[r], sr, zs, nch);
+      else
+        fp_walk<NB, false, false>(L[r], A[r], sr, zs, nch);
     }
   }
 
-  // span-1 -> span-N reduction in LDS
+  // per-LOR values -> LDS; each output bin then sums its LORs in CSR order
+  float *lorv = lds;
+  float sf[FP_ROUNDS];
+  bool have[FP_ROUNDS], mir[FP_ROUNDS];
 #pragma unroll
-  for (int r = 0; r < LI_ROUNDS; r++) {
-    int li = r * PRJ_THREADS + tid;
-    if (li < g.nli) {
-      A[r].fold();
-      float2 z = g.li_z[li];
-      short2 sn = g.li_sn[li];
-      float sf = lor_sfac(z.y, g.vox_z, atn);
-      float nosi = g.li_nosi[li];
+  for (int r = 0; r < FP_ROUNDS; r++) {
+    const int li = r * FP_THREADS + tid;
+    have[r] = li < g.nli;
+    const float2 z = have[r] ? g.li_z[li] : make_float2(0.5f, 0.f);
+    mir[r] = z.y > 0.f;
+    sf[r] = lor_sfac(z.y, g.vox_z, atn);
+    A[r].fold();
+  }
+  static_for<NB>([&](auto bc) {
+    constexpr int b = decltype(bc)::value;
+    __syncthreads();
 #pragma unroll
-      for (int b = 0; b < NB; b++) {
-        float va = A[r].aa[b] * sf, vb = A[r].ab[b] * sf;
+    for (int r = 0; r < FP_ROUNDS; r++) {
+      const int li = r * FP_THREADS + tid;
+      if (have[r]) {
+        float va = A[r].ta[b] * sf[r], vb = A[r].tb[b] * sf[r];
         if (MODE == 1) {
+          const float nosi = g.li_nosi[li];
           va = expf(-va) * nosi;
           vb = expf(-vb) * nosi;
         }
-        atomicAdd(&bins[sn.x * NB + b], va);
-        if (z.y > 0.f) atomicAdd(&bins[sn.y * NB + b], vb);
+        lorv[2 * li] = va;
+        if (mir[r]) lorv[2 * li + 1] = vb;
       }
     }
-  }
-  __syncthreads();
-  for (int i = tid; i < nsn * NB; i += PRJ_THREADS) {
-    const int s = i / NB, b = i - s * NB;
-    float v = bins[i];
-    if (MODE == 2) {
-      const size_t k = (size_t)aw * nsn + s;
-      const float e = v + (rho_nb ? rho[k * nbt + b0 + b] : rho[k]);
-      v = (e > -EPS && e < EPS) ? 0.f : (float)prompts[k * nbt + b0 + b] / e;
+    __syncthreads();
+    for (int s = tid; s < nsn; s += FP_THREADS) {
+      float v = 0.f;
+      const int m1 = g.sn_off[s + 1];
+      for (int m = g.sn_off[s]; m < m1; m++) v += lorv[g.sn_mem[m]];
+      if (MODE == 2) {
+        const size_t kk = (size_t)aw * nsn + s;
+        const float ee = v + (rho_nb ? rho[kk * nbt + b0 + b] : rho[kk]);
+        v = (ee > -EPS && ee < EPS) ? 0.f : (float)prompts[kk * nbt + b0 + b] / ee;
+      }
+      out[(size_t)s * nbt + b] = v;
     }
-    out[(size_t)s * nbt + b] = v;
-  }
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -341,10 +395,6 @@ __global__ __launch_bounds__(PRJ_THREADS) void k_fprj(GeomView g, const float *_
 // each wave flushes its segments through a private LDS buffer with one
 // coalesced global float atomic per (segment, plane, replicate).
 __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
-
-__device__ inline float rdlane(float v, int l) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
 
 template <int NB>
 constexpr int bp_spw() { return NB == 1 ? 8 : 4; }
@@ -411,8 +461,8 @@ __global__ __launch_bounds__(BP_THREADS) void k_bprj(GeomView g, float *__restri
     // lane d: base plane and pieces (times the 3D factor) of each segment
     float P1[SPW], P2[SPW], P3[SPW], L[SPW];
     int W0[SPW], C[SPW];
-#pragma unroll
-    for (int s = 0; s < SPW; s++) {
+    static_for<SPW>([&](auto sc) {
+      constexpr int s = decltype(sc)::value;
       const int k = g0 + s;
       SegRec sg;
       if (k < nseg) {
@@ -435,28 +485,27 @@ __global__ __launch_bounds__(BP_THREADS) void k_bprj(GeomView g, float *__restri
         P1[s] = p1 * sf_lane;
         P2[s] = (c2 - p1) * sf_lane;
         P3[s] = (sg.len - c2) * sf_lane;
-        W0[s] = (int)w0;
+        // base plane (bits 0-7) and which of pieces 2 / 3 are non-zero (bits 8, 9)
+        W0[s] = (int)w0 | ((int)(P2[s] != 0.f) << 8) | ((int)(P3[s] != 0.f) << 9);
       }
-    }
+    });
     float acc0[SPW][NB], acc1[SPW][NB];
-#pragma unroll
-    for (int s = 0; s < SPW; s++)
+    static_for<SPW>([&](auto sc) {
+      constexpr int s = decltype(sc)::value;
 #pragma unroll
       for (int b = 0; b < NB; b++) {
         acc0[s][b] = L[s] * X0[b];
         acc1[s][b] = 0.f;
       }
-    bool any3 = false;
-#pragma unroll
-    for (int s = 0; s < SPW; s++) any3 |= __any(P3[s] != 0.f);
-
+    });
     for (int d = 1; d <= dmax; d++) {
       const int rb = bp_row(d) + 2;
       float S1[NB], S3[NB], Te[NB], To[NB];
       int wprev = -1000;
-#pragma unroll
-      for (int s = 0; s < SPW; s++) {
-        const int w0 = __builtin_amdgcn_readlane(W0[s], d);
+      static_for<SPW>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        const int wv = __builtin_amdgcn_readlane(W0[s], d);
+        const int w0 = wv & 0xff;
         if (w0 != wprev) {  // wave-uniform: new base plane -> reload the LOR values
           wprev = w0;
           const int h = w0 >> 1, e = (2 * d - w0) >> 1;
@@ -476,21 +525,25 @@ __global__ __launch_bounds__(BP_THREADS) void k_bprj(GeomView g, float *__restri
             To[b] = Y[b] + Xb[b];
           }
         }
-        const float p1 = rdlane(P1[s], d), p2 = rdlane(P2[s], d);
+        const float p1 = rdlane(P1[s], d);
         if (w0 & 1) {
 #pragma unroll
-          for (int b = 0; b < NB; b++) {
-            acc0[s][b] = fmaf(p2, To[b], acc0[s][b]);
-            acc1[s][b] = fmaf(p1, S1[b], acc1[s][b]);
-          }
+          for (int b = 0; b < NB; b++) acc1[s][b] = fmaf(p1, S1[b], acc1[s][b]);
         } else {
 #pragma unroll
-          for (int b = 0; b < NB; b++) {
-            acc0[s][b] = fmaf(p1, S1[b], acc0[s][b]);
-            acc1[s][b] = fmaf(p2, Te[b], acc1[s][b]);
+          for (int b = 0; b < NB; b++) acc0[s][b] = fmaf(p1, S1[b], acc0[s][b]);
+        }
+        if (wv & 0x100) {  // a z boundary inside the segment (zero pieces are skipped)
+          const float p2 = rdlane(P2[s], d);
+          if (w0 & 1) {
+#pragma unroll
+            for (int b = 0; b < NB; b++) acc0[s][b] = fmaf(p2, To[b], acc0[s][b]);
+          } else {
+#pragma unroll
+            for (int b = 0; b < NB; b++) acc1[s][b] = fmaf(p2, Te[b], acc1[s][b]);
           }
         }
-        if (any3) {
+        if (wv & 0x200) {
           const float p3 = rdlane(P3[s], d);
           if (w0 & 1) {
 #pragma unroll
@@ -500,12 +553,12 @@ __global__ __launch_bounds__(BP_THREADS) void k_bprj(GeomView g, float *__restri
             for (int b = 0; b < NB; b++) acc0[s][b] = fmaf(p3, S3[b], acc0[s][b]);
           }
         }
-      }
+      });
     }
     // flush the wave's segments through its LDS buffer (wave-local, no barrier)
-#pragma unroll
-    for (int s = 0; s < SPW; s++) {
-      if (g0 + s >= nseg) break;
+    static_for<SPW>([&](auto sc) {
+      constexpr int s = decltype(sc)::value;
+      if (g0 + s >= nseg) return;
 #pragma unroll
       for (int b = 0; b < NB; b++) {
         wb[(2 * q) * NB + b] = acc0[s][b];
@@ -520,7 +573,7 @@ __global__ __launch_bounds__(BP_THREADS) void k_bprj(GeomView g, float *__restri
         if (v != 0.f) unsafeAtomicAdd(img + ((int64_t)colz + z) * nbt + b0 + b, v);
       }
       __builtin_amdgcn_wave_barrier();
-    }
+    });
   }
 }
 
@@ -575,6 +628,22 @@ __global__ void k_remgaps(float *out, const T *in, const int32_t *aw2ali, int na
   }
 }
 
+// Siemens histogram with replicates innermost, in[(s * nbinang + ali) * nrep + r]
+// (uint32 counts) -> device prompts out[(aw * nsn + s) * nb + b] (uint16,
+// saturating at 65535) for replicates r = r0 + b, b < nb.
+__global__ void k_remgaps_u16b(uint16_t *out, const uint32_t *in, const int32_t *aw2ali, int naw,
+                               int nsn, int nbinang, int nrep, int r0, int nb) {
+  const int64_t n = (int64_t)naw * nsn * nb;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int64_t t = i / nb;
+    const int b = (int)(i - t * nb);
+    const int aw = (int)(t / nsn), s = (int)(t - (int64_t)aw * nsn);
+    const uint32_t v = in[((int64_t)s * nbinang + aw2ali[aw]) * nrep + r0 + b];
+    out[i] = (uint16_t)(v > 65535u ? 65535u : v);
+  }
+}
+
 // device (naw, nsn) -> Siemens (nsn, 252*344) scatter, gaps stay zero
 // (d_putgaps, src/scanner_0.cu:158-166, plus the transpose of mmraux.py:1141).
 __global__ void k_putgaps(float *out, const float *in, const int32_t *aw2ali, int naw, int nsn,
@@ -598,23 +667,28 @@ static int ew_blocks(int64_t n) {
 using namespace nipet;
 
 // Launch a projector over nb replicates as sub-batches of 4, 2 and 1.
+// Largest replicate sub-batch per kernel instance (4, 2 or 1), set by the
+// autotuner through nipet_tune_set; process-wide.
+static std::atomic<int> g_width_fp{4}, g_width_bp{4};
+
 template <int MODE>
 static void launch_fprj(const Geom *g, float *sino, const float *img, const int32_t *subs,
                         int nprj, const uint16_t *prompts, const float *rho, int rho_nb, int nb,
                         hipStream_t st) {
+  const int wmax = g_width_fp.load();
   for (int b0 = 0; b0 < nb;) {
     const int rem = nb - b0;
     const Batch bt{nb, b0};
-    if (rem >= 4) {
-      k_fprj<MODE, 4><<<nprj, PRJ_THREADS, 0, st>>>(view(g), img, sino, subs, prompts, rho,
+    if (rem >= 4 && wmax >= 4) {
+      k_fprj<MODE, 4><<<nprj, FP_THREADS, 0, st>>>(view(g), img, sino, subs, prompts, rho,
                                                    rho_nb, bt);
       b0 += 4;
-    } else if (rem >= 2) {
-      k_fprj<MODE, 2><<<nprj, PRJ_THREADS, 0, st>>>(view(g), img, sino, subs, prompts, rho,
+    } else if (rem >= 2 && wmax >= 2) {
+      k_fprj<MODE, 2><<<nprj, FP_THREADS, 0, st>>>(view(g), img, sino, subs, prompts, rho,
                                                    rho_nb, bt);
       b0 += 2;
     } else {
-      k_fprj<MODE, 1><<<nprj, PRJ_THREADS, 0, st>>>(view(g), img, sino, subs, prompts, rho,
+      k_fprj<MODE, 1><<<nprj, FP_THREADS, 0, st>>>(view(g), img, sino, subs, prompts, rho,
                                                    rho_nb, bt);
       b0 += 1;
     }
@@ -623,13 +697,14 @@ static void launch_fprj(const Geom *g, float *sino, const float *img, const int3
 
 static void launch_bprj(const Geom *g, float *img, const float *sino, const int32_t *subs,
                         int nprj, int nb, hipStream_t st) {
+  const int wmax = g_width_bp.load();
   for (int b0 = 0; b0 < nb;) {
     const int rem = nb - b0;
     const Batch bt{nb, b0};
-    if (rem >= 4) {
+    if (rem >= 4 && wmax >= 4) {
       k_bprj<4><<<nprj, BP_THREADS, 0, st>>>(view(g), img, sino, subs, bt);
       b0 += 4;
-    } else if (rem >= 2) {
+    } else if (rem >= 2 && wmax >= 2) {
       k_bprj<2><<<nprj, BP_THREADS, 0, st>>>(view(g), img, sino, subs, bt);
       b0 += 2;
     } else {
@@ -736,6 +811,40 @@ int nipet_remgaps(float *out, const void *in, int in_dtype, const int32_t *aw2al
     k_remgaps<uint16_t><<<ew_blocks(n), 256, 0, st>>>(out, (const uint16_t *)in, aw2ali, naw,
                                                       nsn, nbinang);
   NIPET_HIP(hipGetLastError());
+  return NIPET_OK;
+}
+
+int nipet_remgaps_u16b(uint16_t *out, const uint32_t *in, const int32_t *aw2ali, int naw,
+                       int nsn, int nbinang, int nrep, int r0, int nb, void *stream) {
+  if (!out || !in || !aw2ali || naw <= 0 || nsn <= 0 || nbinang <= 0 || nrep < 1 || r0 < 0 ||
+      nb < 1 || r0 + nb > nrep) {
+    set_error("nipet_remgaps_u16b: bad arguments (naw=%d nsn=%d nrep=%d r0=%d nb=%d)", naw, nsn,
+              nrep, r0, nb);
+    return NIPET_EINVAL;
+  }
+  const int64_t n = (int64_t)naw * nsn * nb;
+  k_remgaps_u16b<<<ew_blocks(n), 256, 0, (hipStream_t)stream>>>(out, in, aw2ali, naw, nsn,
+                                                               nbinang, nrep, r0, nb);
+  NIPET_HIP(hipGetLastError());
+  return NIPET_OK;
+}
+
+int nipet_tune_set(int kernel, int width) {
+  if ((kernel != NIPET_PROF_FPRJ && kernel != NIPET_PROF_BPRJ) ||
+      (width != 1 && width != 2 && width != 4)) {
+    set_error("nipet_tune_set: kernel must be FPRJ or BPRJ and width 1, 2 or 4");
+    return NIPET_EINVAL;
+  }
+  (kernel == NIPET_PROF_FPRJ ? g_width_fp : g_width_bp).store(width);
+  return NIPET_OK;
+}
+
+int nipet_tune_get(int kernel, int *width) {
+  if ((kernel != NIPET_PROF_FPRJ && kernel != NIPET_PROF_BPRJ) || !width) {
+    set_error("nipet_tune_get: kernel must be FPRJ or BPRJ");
+    return NIPET_EINVAL;
+  }
+  *width = (kernel == NIPET_PROF_FPRJ ? g_width_fp : g_width_bp).load();
   return NIPET_OK;
 }
 

@@ -69,7 +69,7 @@ class LmHistogram:
     """Device histogram of one frame of an mMR list-mode word stream."""
 
     def __init__(self, scanner_params, toff, nitag, tstart=0, tstop=None, span=None, btp=None,
-                 btprt=None, seed=0, nrep=1):
+                 btprt=None, seed=0, nrep=1, rep0=0):
         torch = _torch()
         Cnt, tx, ax = scanner_params['Cnt'], scanner_params['txLUT'], scanner_params['axLUT']
         self.span = Cnt['SPN'] if span is None else span
@@ -84,7 +84,7 @@ class LmHistogram:
                       np.ascontiguousarray(tx['s2cF'], np.int16)]
         cfg = _lib.NipetLmConfig(int(toff), self.nitag, int(tstart), int(tstop), self.span,
                                  int(self.btp), float(self.btprt), int(seed), self.nrep,
-                                 *[k.ctypes.data for k in self._keep])
+                                 int(rep0), *[k.ctypes.data for k in self._keep])
         h = ct.c_void_p()
         _lib.check(_lib.lib().nipet_lm_create(ct.byref(cfg), ct.byref(h)), 'nipet_lm_create')
         self.handle = h
@@ -131,18 +131,32 @@ class LmHistogram:
         return n.value, bad.value, ovf.value
 
 
-def hist_words(words, scanner_params, t0=0, t1=0, btp=None, seed=0, nrep=1, span=None):
+def tag_range(words):
+    """(first, last) time-tag values [ms] of a word array (CUDA tensor: found
+    on the device; else NumPy)."""
+    torch = _torch()
+    if isinstance(words, torch.Tensor) and words.is_cuda:
+        idx = torch.nonzero((words >> 29) == -4).squeeze(1)
+        if idx.numel() == 0:
+            raise ValueError('no time tags in the list-mode data')
+        ends = words[idx[[0, -1]]].cpu().numpy()
+        return int(ends[0] & 0x1fffffff), int(ends[1] & 0x1fffffff)
+    w = words.numpy() if isinstance(words, torch.Tensor) else np.asarray(words)
+    (tf, _), (tl, _) = _tag_positions(w)
+    return tf, tl
+
+
+def hist_words(words, scanner_params, t0=0, t1=0, btp=None, seed=0, nrep=1, span=None, rep0=0):
     """Histogram a device (or host) int32 word array that holds a complete
     acquisition. Returns the ``LmHistogram`` (device outputs in ``.out``)."""
     torch = _torch()
     w = words if isinstance(words, torch.Tensor) else torch.from_numpy(np.asarray(words))
-    wh = w.cpu().numpy() if w.is_cuda else w.numpy()
-    (tf, _), (tl, _) = _tag_positions(wh)
+    tf, tl = tag_range(w)
     nitag = ((tl - tf) + 999) // 1000
     if t0 == t1:
         t0, t1 = 0, nitag
     h = LmHistogram(scanner_params, tf, nitag, t0, min(t1, nitag), span=span, btp=btp, seed=seed,
-                    nrep=nrep)
+                    nrep=nrep, rep0=rep0)
     h.add(w.to('cuda', torch.int32))
     h.finish()
     return h

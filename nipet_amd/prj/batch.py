@@ -1,0 +1,199 @@
+"""Batched reconstruction of bootstrap replicates and dynamic frames.
+
+The reference reconstructs frames and bootstrap realisations one after the
+other, each with its own list-mode pass and ``osemone`` call (``mmrchain``,
+img/pipe.py:295-373; bootstrap weights in the histogrammer,
+lm/src/hst.cu:110-139). Here nb reconstructions that share attenuation and
+normalisation (hence the 14 sensitivity images) run as one batch:
+
+* one list-mode pass histograms all bootstrap replicates (replicate-innermost
+  counts, Philox counter = (word index, replicate), so replicate r is the same
+  whichever session produces it), and ``nipet_remgaps_u16b`` turns them into
+  the batched uint16 prompts (Naw, 837, nb);
+* dynamic frames take one histogram session per time window and carry their
+  own randoms + scatter (rho per frame);
+* the whole ``itr`` x 14 OSEM of all nb images is captured once as a HIP
+  graph (``torch.cuda.CUDAGraph`` over the library's stream launches: no host
+  synchronisation, allocation or memset inside) and runs as one graph launch;
+* the replicate sub-batch width of the FP and BP kernels (4, 2 or 1 images per
+  launch) is picked by timing both kernels on the problem's own geometry and
+  batch size (``autotune``).
+"""
+import numpy as np
+
+from .. import _lib
+from .osem import OsemBatch, osem_iteration_batch, osem_prepare_batch
+
+__all__ = ['autotune', 'OsemGraph', 'bootstrap_prompts', 'frame_prompts', 'bootstrap_osem',
+           'frames_osem']
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def autotune(state, widths=(4, 2, 1), repeats=2):
+    """Time the FP (OSEM ratio mode 0 here: emission) and BP kernels of
+    ``state`` (an ``OsemBatch``) on its first subset for each sub-batch width
+    and keep the fastest for each kernel. Returns {'fprj': w, 'bprj': w,
+    'ms': {(kernel, w): ms per launch}}."""
+    torch = _torch()
+    L = _lib.lib()
+    st = _lib.stream_handle()
+    subs = state.subsets[0]
+    nprj = len(subs)
+    img = torch.ones((320, 320, 127, state.nb), dtype=torch.float32, device='cuda')
+    res = {'ms': {}}
+    for kern, name in ((_lib.PROF_FPRJ, 'fprj'), (_lib.PROF_BPRJ, 'bprj')):
+        def launch():
+            if kern == _lib.PROF_FPRJ:
+                rc = L.nipet_fprj_b(state.geom.handle, _lib.ptr(state.work_sino), _lib.ptr(img),
+                                    _lib.ptr(subs), nprj, state.nb, 0, st)
+            else:
+                rc = L.nipet_bprj_b(state.geom.handle, _lib.ptr(state.work_img),
+                                    _lib.ptr(state.work_sino), _lib.ptr(subs), nprj, state.nb,
+                                    st)
+            _lib.check(rc, name)
+        best = None
+        for w in widths:
+            _lib.check(L.nipet_tune_set(kern, int(w)), 'nipet_tune_set')
+            launch()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(repeats):
+                launch()
+            e1.record()
+            e1.synchronize()
+            ms = e0.elapsed_time(e1) / repeats
+            res['ms'][(name, w)] = ms
+            if best is None or ms < best[1]:
+                best = (w, ms)
+        _lib.check(L.nipet_tune_set(kern, best[0]), 'nipet_tune_set')
+        res[name] = best[0]
+    return res
+
+
+class OsemGraph:
+    """``niter`` batched OSEM iterations of ``state`` captured as one HIP graph
+    on a persistent image buffer [320, 320, 127, nb]; ``run`` resets the
+    image (ones, or ``img0``) and launches the graph."""
+
+    def __init__(self, state, niter):
+        torch = _torch()
+        if not isinstance(state, OsemBatch):
+            raise TypeError('state must be an OsemBatch')
+        self.state, self.niter = state, int(niter)
+        self.img = torch.ones((320, 320, 127, state.nb), dtype=torch.float32, device='cuda')
+        _lib.check(_lib.lib().nipet_prof_enable(0), 'nipet_prof_enable')
+        # load every kernel instance before capture: one subset update over a
+        # single transaxial bin (the image is reset before each run)
+        first = state.subsets[0][:1].contiguous()
+        _lib.check(_lib.lib().nipet_osem_subset_b(
+            state.geom.handle, _lib.ptr(self.img), _lib.ptr(state.prompts),
+            _lib.ptr(state.rho), int(state.rho_per_rep), _lib.ptr(state.sens[0]), 0,
+            _lib.ptr(state.mask), _lib.ptr(first), 1, state.nb, _lib.ptr(state.work_sino),
+            _lib.ptr(state.work_img), _lib.stream_handle()), 'nipet_osem_subset_b')
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            for _ in range(self.niter):
+                osem_iteration_batch(state, self.img)
+        torch.cuda.synchronize()
+
+    def run(self, img0=None):
+        if img0 is None:
+            self.img.fill_(1.0)
+        else:
+            self.img.copy_(img0)
+        self.graph.replay()
+        return self.img
+
+
+def bootstrap_prompts(words, scanner_params, nrep, seed, rep0=0, t0=0, t1=0, hist=None):
+    """Batched uint16 prompts (Naw, 837, nrep) of the bootstrap replicates
+    rep0 .. rep0+nrep-1 of the frame [t0, t1) of a device word array (one
+    list-mode pass). Returns (prompts, LmHistogram)."""
+    torch = _torch()
+    from ..lm.mmrhist import hist_words
+    h = hist if hist is not None else hist_words(words, scanner_params, t0, t1, btp=2, seed=seed,
+                                                 nrep=nrep, rep0=rep0)
+    return _remgaps_b(h, scanner_params, h.nrep, 0, h.nrep), h
+
+
+def _remgaps_b(h, scanner_params, nrep, r0, nb):
+    torch = _torch()
+    ali = torch.from_numpy(np.asarray(scanner_params['txLUT']['aw2ali'], np.int32)).cuda()
+    naw = ali.numel()
+    nsn = h.nsinos
+    out = torch.empty((naw, nsn, nb), dtype=torch.uint16, device='cuda')
+    _lib.check(_lib.lib().nipet_remgaps_u16b(_lib.ptr(out), _lib.ptr(h.out['psino']),
+                                             _lib.ptr(ali), naw, nsn, 86688, nrep, r0, nb,
+                                             _lib.stream_handle()), 'nipet_remgaps_u16b')
+    return out
+
+
+def frame_prompts(words, scanner_params, frames):
+    """Batched uint16 prompts (Naw, 837, nf) of the time frames [(t0, t1), ...]
+    (seconds from the first tag) of a device word array, one histogram session
+    per frame over the words of its window. Returns (prompts, [delayed totals])."""
+    torch = _torch()
+    from ..lm.mmrhist import LmHistogram, tag_range
+    tf, tl = tag_range(words)
+    nitag = ((tl - tf) + 999) // 1000
+    tags = torch.nonzero((words >> 29) == -4).squeeze(1)
+    tms = ((words[tags] & 0x1fffffff) - tf).to(torch.int64)
+    out = []
+    dtot = []
+    for t0, t1 in frames:
+        if not (0 <= t0 < t1 <= nitag):
+            raise ValueError(f'frame ({t0}, {t1}) outside 0..{nitag} s')
+        # words from the first tag of second t0 to the first tag of second t1
+        i0 = int(tags[int(torch.searchsorted(tms, t0 * 1000))])
+        k1 = int(torch.searchsorted(tms, t1 * 1000))
+        i1 = int(tags[k1]) if k1 < tags.numel() else words.numel()
+        h = LmHistogram(scanner_params, tf, nitag, t0, t1, btp=0)
+        h.add(words[i0:i1])
+        out.append(_remgaps_b(h, scanner_params, 1, 0, 1))
+        dtot.append(int(h.out['dsino'].sum(dtype=torch.int64)))
+        del h
+    return torch.cat(out, dim=2).contiguous(), dtot
+
+
+def bootstrap_osem(words, scanner_params, asng, nsng, rsng=None, ssng=None, nrep=64, seed=1,
+                   rep0=0, itr=4, t0=0, t1=0, tune=True, graph=True, sens=None):
+    """OSEM (``itr`` x 14) of bootstrap replicates rep0 .. rep0+nrep-1 of a
+    device list-mode word array, batched; corrections (Naw, 837) shared by all
+    replicates. Returns (images [320, 320, 127, nrep], info)."""
+    torch = _torch()
+    prompts, h = bootstrap_prompts(words, scanner_params, nrep, seed, rep0, t0, t1)
+    del h
+    st = osem_prepare_batch(scanner_params, prompts, asng, nsng, rsng, ssng, sens=sens)
+    info = {'tune': autotune(st) if tune else None}
+    if graph:
+        g = OsemGraph(st, itr)
+        img = g.run().clone()
+    else:
+        img = torch.ones((320, 320, 127, st.nb), dtype=torch.float32, device='cuda')
+        for _ in range(itr):
+            osem_iteration_batch(st, img)
+    info['sens'] = st.sens
+    return img, info
+
+
+def frames_osem(words, scanner_params, frames, asng, nsng, rho_frames, itr=4, tune=True,
+                graph=True, sens=None):
+    """OSEM of dynamic frames [(t0, t1), ...] of a device word array, batched;
+    attenuation / norm (Naw, 837) shared, ``rho_frames`` = randoms + scatter
+    per frame (Naw, 837, nf). Returns (images [320, 320, 127, nf], info)."""
+    torch = _torch()
+    prompts, dtot = frame_prompts(words, scanner_params, frames)
+    st = osem_prepare_batch(scanner_params, prompts, asng, nsng, rho_frames, None, sens=sens)
+    info = {'tune': autotune(st) if tune else None, 'delayed': dtot}
+    if graph:
+        img = OsemGraph(st, itr).run().clone()
+    else:
+        img = torch.ones((320, 320, 127, st.nb), dtype=torch.float32, device='cuda')
+        for _ in range(itr):
+            osem_iteration_batch(st, img)
+    return img, info

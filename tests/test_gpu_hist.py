@@ -125,3 +125,38 @@ def test_mmrhist_file(torch, params, stream, tmp_path):
     assert np.array_equal(hst['dsino'].ravel(), (o['dsino'][0] & 0xffff).astype(np.uint16))
     assert np.array_equal(hst['phc'], o['phc'][0]) and hst['dur'] == 4
     assert hst['fansums'].shape == (64, 504) and hst['pssr'].shape == (127, 252, 344)
+
+
+def test_rep0_selects_replicate(torch, params, stream):
+    """A session with nrep = 1, rep0 = 2 histograms exactly replicate 2 of an
+    nrep = 3 session (oracle), so replicate sets can be split across sessions."""
+    from oracle import oracle as O
+    toff, nitag = _toff_nitag(stream)
+    w = stream[:900_000]
+    _, g = _gpu(torch, params, w, toff, nitag, btp=2, seed=2024, nrep=1, rep0=2)
+    o = O.lm_hist(w, params, toff, nitag, btp=2, seed=2024, nrep=3)
+    assert np.array_equal(g['psino'][:, 0].astype(np.uint32), o['psino'][2])
+    assert np.array_equal(g['phc'][:, 0].astype(np.uint32), o['phc'][2])
+
+
+def test_remgaps_u16b(torch, params, stream):
+    """Replicate-innermost histogram -> batched uint16 prompts (naw, 837, nb)
+    for replicates r0 .. r0+nb-1: equal to the NumPy gather by aw2ali."""
+    import ctypes as ct
+
+    from nipet_amd import _lib
+    toff, nitag = _toff_nitag(stream)
+    h, g = _gpu(torch, params, stream[:900_000], toff, nitag, btp=2, seed=7, nrep=5)
+    ali = np.asarray(params['txLUT']['aw2ali'], np.int32)
+    naw = len(ali)
+    out = torch.empty((naw, 837, 3), dtype=torch.uint16, device='cuda')
+    ali_d = torch.from_numpy(ali).cuda()
+    _lib.check(_lib.lib().nipet_remgaps_u16b(out.data_ptr(), h.out['psino'].data_ptr(),
+                                             ali_d.data_ptr(), naw, 837, 86688, 5, 1, 3,
+                                             _lib.stream_handle()), 'remgaps_u16b')
+    torch.cuda.synchronize()
+    ref = g['psino'].reshape(837, 86688, 5)[:, ali, 1:4].transpose(1, 0, 2)
+    assert np.array_equal(out.cpu().numpy().astype(np.int64), np.minimum(ref, 65535))
+    bad = _lib.lib().nipet_remgaps_u16b(out.data_ptr(), h.out['psino'].data_ptr(),
+                                        ali_d.data_ptr(), naw, 837, 86688, 5, 3, 3, None)
+    assert bad != 0
