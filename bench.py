@@ -166,6 +166,7 @@ def batched_bootstrap(params, d, nrep, nwords, nplain=2):
             'batch_s': round(t_batch, 3), 'plain_s_per_recon': round(tp, 4),
             'plain_checked': nplain, 'max_rel_l2_vs_plain': max(rels),
             'widths': {'fprj': info['tune']['fprj'], 'bprj': info['tune']['bprj']},
+            'batch_breakdown_s': info['timing'],
             'timing': 'wall clock incl. histogramming, gap removal, corrections, '
                       'sensitivity, autotune, graph capture'}
 

@@ -33,27 +33,39 @@ def _torch():
     return torch
 
 
-def autotune(state, widths=(4, 2, 1), repeats=2):
-    """Time the FP (OSEM ratio mode 0 here: emission) and BP kernels of
-    ``state`` (an ``OsemBatch``) on its first subset for each sub-batch width
-    and keep the fastest for each kernel. Returns {'fprj': w, 'bprj': w,
-    'ms': {(kernel, w): ms per launch}}."""
+_TUNED = {}
+
+
+def autotune(state, widths=(4, 2, 1), repeats=2, probe=4):
+    """Time the emission FP and the BP kernels on the first subset of
+    ``state`` (an ``OsemBatch``) for each sub-batch width, over a probe batch
+    of min(nb, ``probe``) replicates, and keep the fastest width per kernel.
+    Results are cached per (geometry, subset size) for the process. Returns
+    {'fprj': w, 'bprj': w, 'ms': {(kernel, w): ms per launch}}."""
     torch = _torch()
     L = _lib.lib()
+    key = (id(state.geom), len(state.subsets[0]))
+    if key in _TUNED:
+        res = _TUNED[key]
+        _lib.check(L.nipet_tune_set(_lib.PROF_FPRJ, res['fprj']), 'nipet_tune_set')
+        _lib.check(L.nipet_tune_set(_lib.PROF_BPRJ, res['bprj']), 'nipet_tune_set')
+        return res
     st = _lib.stream_handle()
     subs = state.subsets[0]
     nprj = len(subs)
-    img = torch.ones((320, 320, 127, state.nb), dtype=torch.float32, device='cuda')
+    nb = min(state.nb, probe)
+    img = torch.ones((320, 320, 127, nb), dtype=torch.float32, device='cuda')
+    sino = torch.empty((nprj, state.geom.nsn, nb), dtype=torch.float32, device='cuda')
+    bimg = torch.empty_like(img)
     res = {'ms': {}}
     for kern, name in ((_lib.PROF_FPRJ, 'fprj'), (_lib.PROF_BPRJ, 'bprj')):
         def launch():
             if kern == _lib.PROF_FPRJ:
-                rc = L.nipet_fprj_b(state.geom.handle, _lib.ptr(state.work_sino), _lib.ptr(img),
-                                    _lib.ptr(subs), nprj, state.nb, 0, st)
+                rc = L.nipet_fprj_b(state.geom.handle, _lib.ptr(sino), _lib.ptr(img),
+                                    _lib.ptr(subs), nprj, nb, 0, st)
             else:
-                rc = L.nipet_bprj_b(state.geom.handle, _lib.ptr(state.work_img),
-                                    _lib.ptr(state.work_sino), _lib.ptr(subs), nprj, state.nb,
-                                    st)
+                rc = L.nipet_bprj_b(state.geom.handle, _lib.ptr(bimg), _lib.ptr(sino),
+                                    _lib.ptr(subs), nprj, nb, st)
             _lib.check(rc, name)
         best = None
         for w in widths:
@@ -71,6 +83,7 @@ def autotune(state, widths=(4, 2, 1), repeats=2):
                 best = (w, ms)
         _lib.check(L.nipet_tune_set(kern, best[0]), 'nipet_tune_set')
         res[name] = best[0]
+    _TUNED[key] = res
     return res
 
 
@@ -165,19 +178,36 @@ def bootstrap_osem(words, scanner_params, asng, nsng, rsng=None, ssng=None, nrep
     """OSEM (``itr`` x 14) of bootstrap replicates rep0 .. rep0+nrep-1 of a
     device list-mode word array, batched; corrections (Naw, 837) shared by all
     replicates. Returns (images [320, 320, 127, nrep], info)."""
+    import time
     torch = _torch()
+    tm = {}
+
+    def mark(name, t0=[None]):
+        torch.cuda.synchronize()
+        now = time.perf_counter()
+        if t0[0] is not None:
+            tm[name] = round(now - t0[0], 4)
+        t0[0] = now
+
+    mark(None)
     prompts, h = bootstrap_prompts(words, scanner_params, nrep, seed, rep0, t0, t1)
     del h
+    mark('histogram_s')
     st = osem_prepare_batch(scanner_params, prompts, asng, nsng, rsng, ssng, sens=sens)
+    mark('prepare_s')
     info = {'tune': autotune(st) if tune else None}
+    mark('tune_s')
     if graph:
         g = OsemGraph(st, itr)
+        mark('capture_s')
         img = g.run().clone()
     else:
         img = torch.ones((320, 320, 127, st.nb), dtype=torch.float32, device='cuda')
         for _ in range(itr):
             osem_iteration_batch(st, img)
+    mark('osem_s')
     info['sens'] = st.sens
+    info['timing'] = tm
     return img, info
 
 
