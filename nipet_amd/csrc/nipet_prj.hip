@@ -48,9 +48,24 @@ struct GeomView {
 };
 
 static GeomView view(const Geom *g) {
-  return GeomView{g->off,   g->seg,    g->atn,    g->lmax, g->li_z,    g->li_sn, g->li_nosi,
-                  g->dr_li, g->sn_off, g->sn_mem, g->naw,  g->nli,     g->ndirect, g->nsn,
-                  g->dmax,  g->vox_z};
+  GeomView v{};
+  v.off = g->off;
+  v.seg = g->seg;
+  v.atn = g->atn;
+  v.lmax = g->lmax;
+  v.li_z = g->li_z;
+  v.li_sn = g->li_sn;
+  v.li_nosi = g->li_nosi;
+  v.dr_li = g->dr_li;
+  v.sn_off = g->sn_off;
+  v.sn_mem = g->sn_mem;
+  v.naw = g->naw;
+  v.nli = g->nli;
+  v.ndirect = g->ndirect;
+  v.nsn = g->nsn;
+  v.dmax = g->dmax;
+  v.vox_z = g->vox_z;
+  return v;
 }
 
 // A sub-batch of NB replicates starting at replicate b0 of a batch of nbt:
@@ -174,7 +189,7 @@ __device__ inline void fp_walk(FpLor &l, FpSum<NB> &acc, const float2 *__restric
   for (int j8 = 0; j8 < nch; j8 += SUBSUM) {
     const float *za = zs + l.ao + j8 * NB;
     const float *zb = zs + l.bo + j8 * NB;
-#pragma unroll(NB >= 4 ? 2 : SUBSUM)
+#pragma unroll(NB >= 4 ? 2 : 4)
     for (int jj = 0; jj < SUBSUM; jj++) {
       const float2 sl = sr[j8 + jj];
       float xa0[NB], xa1[NB];
@@ -231,7 +246,7 @@ __device__ inline void fp_walk(FpLor &l, FpSum<NB> &acc, const float2 *__restric
 }
 
 template <int MODE, int NB>
-__global__ __launch_bounds__(FP_THREADS) void k_fprj(GeomView g, const float *__restrict__ img,
+__global__ __launch_bounds__(FP_THREADS, NB >= 4 ? 4 : 6) void k_fprj(GeomView g, const float *__restrict__ img,
                                                      float *__restrict__ sino,
                                                      const int32_t *__restrict__ subs,
                                                      const uint16_t *__restrict__ prompts,
@@ -242,6 +257,8 @@ __global__ __launch_bounds__(FP_THREADS) void k_fprj(GeomView g, const float *__
   constexpr int ZN = FP_PLANES * PS * NB, LN = 2 * LI_MAX;
   __shared__ __align__(16) float lds[ZN > LN ? ZN : LN];
   __shared__ float2 sr[CH];
+  __shared__ double inv[FP_ROUNDS][FP_THREADS];  // exact boundary spacing (0: direct)
+  __shared__ float z0[FP_ROUNDS][FP_THREADS];    // zeta of LOR a at t = 0
   const int tid = threadIdx.x, wave = tid >> 6;
   const int row = blockIdx.x;
   const int aw = subs ? subs[row] : row;
@@ -263,8 +280,6 @@ __global__ __launch_bounds__(FP_THREADS) void k_fprj(GeomView g, const float *__
 
   FpLor L[FP_ROUNDS];
   FpSum<NB> A[FP_ROUNDS];
-  float z0[FP_ROUNDS];
-  double inv[FP_ROUNDS];
   bool two[FP_ROUNDS], direct[FP_ROUNDS], live[FP_ROUNDS];
   const double s0 = nseg > 0 ? g.seg[beg].s : 0.0;
 #pragma unroll
@@ -273,23 +288,24 @@ __global__ __launch_bounds__(FP_THREADS) void k_fprj(GeomView g, const float *__
     const float2 z = li < g.nli ? g.li_z[li] : make_float2(0.5f, 0.f);
     const int wsum = (int)floorf(z.x) + (int)floorf(z.x + z.y);
     int ia;
-    z0[r] = z.x;
+    double iv = 0.0;
     if (z.y > 0.f) {
-      inv[r] = atn / (double)z.y;
-      ia = (int)floor((double)z.x + s0 / inv[r]);
+      iv = atn / (double)z.y;
+      ia = (int)floor((double)z.x + s0 / iv);
     } else {
-      inv[r] = 0.0;
       ia = (int)floorf(z.x);
     }
+    z0[r][tid] = z.x;
+    inv[r][tid] = iv;
     L[r].ao = (ia + FP_GUARD) * PS * NB;
     L[r].bo = (wsum - ia - 2 + FP_GUARD) * PS * NB;
-    L[r].invf = (float)inv[r];
+    L[r].invf = (float)iv;
 #pragma unroll
     for (int b = 0; b < NB; b++) A[r].pa[b] = A[r].pb[b] = A[r].ta[b] = A[r].tb[b] = 0.f;
     const int li0 = r * FP_THREADS + wave * 64;  // wave-uniform
     live[r] = li0 < g.nli;
     direct[r] = li0 + 64 <= g.ndirect;
-    two[r] = __any(inv[r] > 0.0 && inv[r] < (double)lmax);
+    two[r] = __any(iv > 0.0 && iv < (double)lmax);
   }
 
   for (int c = 0; c < nseg; c += CH) {
@@ -311,7 +327,8 @@ __global__ __launch_bounds__(FP_THREADS) void k_fprj(GeomView g, const float *__
 #pragma unroll
     for (int r = 0; r < FP_ROUNDS; r++) {
       const int ia = L[r].ao / (PS * NB) - FP_GUARD;
-      L[r].tbr = inv[r] > 0.0 ? (float)(((double)(ia + 1) - (double)z0[r]) * inv[r] - sb) : 3e38f;
+      const double iv = inv[r][tid];
+      L[r].tbr = iv > 0.0 ? (float)(((double)(ia + 1) - (double)z0[r][tid]) * iv - sb) : 3e38f;
     }
     __syncthreads();
 #pragma unroll
@@ -344,16 +361,16 @@ __global__ __launch_bounds__(FP_THREADS) void k_fprj(GeomView g, const float *__
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < FP_ROUNDS; r++) {
-      const int li = r * FP_THREADS + tid;
+      const int lv = r * FP_THREADS + tid;  // li
       if (have[r]) {
         float va = A[r].ta[b] * sf[r], vb = A[r].tb[b] * sf[r];
         if (MODE == 1) {
-          const float nosi = g.li_nosi[li];
+          const float nosi = g.li_nosi[lv];
           va = expf(-va) * nosi;
           vb = expf(-vb) * nosi;
         }
-        lorv[2 * li] = va;
-        if (mir[r]) lorv[2 * li + 1] = vb;
+        lorv[2 * lv] = va;
+        if (mir[r]) lorv[2 * lv + 1] = vb;
       }
     }
     __syncthreads();
