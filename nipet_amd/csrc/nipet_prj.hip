@@ -422,7 +422,7 @@ template <int NB>
 constexpr int bp_spw() { return NB == 1 ? 8 : 4; }
 
 template <int NB>
-__global__ __launch_bounds__(BP_THREADS) void k_bprj(GeomView g, float *__restrict__ img,
+__global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g, float *__restrict__ img,
                                                      const float *__restrict__ sino,
                                                      const int32_t *__restrict__ subs, Batch bt) {
   constexpr int SPW = bp_spw<NB>();
@@ -483,6 +483,7 @@ __global__ __launch_bounds__(BP_THREADS) void k_bprj(GeomView g, float *__restri
     // lane d: base plane and pieces (times the 3D factor) of each segment
     float P1[SPW], P2[SPW], P3[SPW], L[SPW];
     int W0[SPW], C[SPW];
+    int wlast = -1;
     static_for<SPW>([&](auto sc) {
       constexpr int s = decltype(sc)::value;
       const int k = g0 + s;
@@ -507,8 +508,13 @@ __global__ __launch_bounds__(BP_THREADS) void k_bprj(GeomView g, float *__restri
         P1[s] = p1 * sf_lane;
         P2[s] = (c2 - p1) * sf_lane;
         P3[s] = (sg.len - c2) * sf_lane;
-        // base plane (bits 0-7) and which of pieces 2 / 3 are non-zero (bits 8, 9)
+        // base plane (bits 0-7), pieces 2 / 3 non-zero (bits 8, 9), base plane
+        // differs from the previous segment's: the LOR values change (bit 10)
         W0[s] = (int)w0 | ((int)(P2[s] != 0.f) << 8) | ((int)(P3[s] != 0.f) << 9);
+        if constexpr (NB > 1) {
+          W0[s] |= (int)(s == 0 || (int)w0 != wlast) << 10;
+          wlast = (int)w0;
+        }
       }
     });
     float acc0[SPW][NB], acc1[SPW][NB];
@@ -520,62 +526,118 @@ __global__ __launch_bounds__(BP_THREADS) void k_bprj(GeomView g, float *__restri
         acc1[s][b] = 0.f;
       }
     });
-    for (int d = 1; d <= dmax; d++) {
-      const int rb = bp_row(d) + 2;
-      float S1[NB], S3[NB], Te[NB], To[NB];
-      int wprev = -1000;
-      static_for<SPW>([&](auto sc) {
-        constexpr int s = decltype(sc)::value;
-        const int wv = __builtin_amdgcn_readlane(W0[s], d);
-        const int w0 = wv & 0xff;
-        if (w0 != wprev) {  // wave-uniform: new base plane -> reload the LOR values
-          wprev = w0;
-          const int h = w0 >> 1, e = (2 * d - w0) >> 1;
-          const int ka = rb + clampi(q - h, -1, 65 - d);
-          const int kb = rb + clampi(q - e, -2, 64 - d);
-          const int ta = pa[ka], tbb = pb[kb];
-          float X[NB], Y[NB], Xb[NB], Yb[NB];
-          ldv<NB>(X, srow + (ta & 0xffff) * NB);
-          ldv<NB>(Y, srow + (ta >> 16) * NB);
-          ldv<NB>(Xb, srow + (tbb & 0xffff) * NB);
-          ldv<NB>(Yb, srow + (tbb >> 16) * NB);
-#pragma unroll
+    if constexpr (NB == 1) {
+      // one image: parity by branch, flags tested one by one (fewer
+      // registers, three workgroups per CU)
+      for (int d = 1; d <= dmax; d++) {
+        const int rb = bp_row(d) + 2;
+        float S1[NB], S3[NB], Te[NB], To[NB];
+        int wprev = -1000;
+        static_for<SPW>([&](auto sc) {
+          constexpr int s = decltype(sc)::value;
+          const int wv = __builtin_amdgcn_readlane(W0[s], d);
+          const int w0 = wv & 0xff;
+          if (w0 != wprev) {  // wave-uniform: new base plane -> reload the LOR values
+            wprev = w0;
+            const int h = w0 >> 1, e = (2 * d - w0) >> 1;
+            const int ka = rb + clampi(q - h, -1, 65 - d);
+            const int kb = rb + clampi(q - e, -2, 64 - d);
+            const int ta = pa[ka], tbb = pb[kb];
+            float X[NB], Y[NB], Xb[NB], Yb[NB];
+            ldv<NB>(X, srow + (ta & 0xffff) * NB);
+            ldv<NB>(Y, srow + (ta >> 16) * NB);
+            ldv<NB>(Xb, srow + (tbb & 0xffff) * NB);
+            ldv<NB>(Yb, srow + (tbb >> 16) * NB);
+  #pragma unroll
+            for (int b = 0; b < NB; b++) {
+              S1[b] = X[b] + Xb[b];
+              S3[b] = Y[b] + Yb[b];
+              Te[b] = X[b] + Yb[b];
+              To[b] = Y[b] + Xb[b];
+            }
+          }
+          const float p1 = rdlane(P1[s], d);
+          if (w0 & 1) {
+  #pragma unroll
+            for (int b = 0; b < NB; b++) acc1[s][b] = fmaf(p1, S1[b], acc1[s][b]);
+          } else {
+  #pragma unroll
+            for (int b = 0; b < NB; b++) acc0[s][b] = fmaf(p1, S1[b], acc0[s][b]);
+          }
+          if (wv & 0x100) {  // a z boundary inside the segment (zero pieces are skipped)
+            const float p2 = rdlane(P2[s], d);
+            if (w0 & 1) {
+  #pragma unroll
+              for (int b = 0; b < NB; b++) acc0[s][b] = fmaf(p2, To[b], acc0[s][b]);
+            } else {
+  #pragma unroll
+              for (int b = 0; b < NB; b++) acc1[s][b] = fmaf(p2, Te[b], acc1[s][b]);
+            }
+          }
+          if (wv & 0x200) {
+            const float p3 = rdlane(P3[s], d);
+            if (w0 & 1) {
+  #pragma unroll
+              for (int b = 0; b < NB; b++) acc1[s][b] = fmaf(p3, S3[b], acc1[s][b]);
+            } else {
+  #pragma unroll
+              for (int b = 0; b < NB; b++) acc0[s][b] = fmaf(p3, S3[b], acc0[s][b]);
+            }
+          }
+        });
+      }
+    } else {
+      // batches: one flag test per step, scalar-selected coefficients
+      for (int d = 1; d <= dmax; d++) {
+        const int rb = bp_row(d) + 2;
+        float S1[NB], S3[NB], Te[NB], To[NB];
+        static_for<SPW>([&](auto sc) {
+          constexpr int s = decltype(sc)::value;
+          const int wv = __builtin_amdgcn_readlane(W0[s], d);
+          const float p1 = rdlane(P1[s], d);
+          const bool odd = wv & 1;
+          if (wv & 0x700) {  // wave-uniform, about one step in three
+            if (wv & 0x400) {  // new base plane: reload the LOR values
+              const int w0 = wv & 0xff;
+              const int h = w0 >> 1, e = (2 * d - w0) >> 1;
+              const int ka = rb + clampi(q - h, -1, 65 - d);
+              const int kb = rb + clampi(q - e, -2, 64 - d);
+              const int ta = pa[ka], tbb = pb[kb];
+              float X[NB], Y[NB], Xb[NB], Yb[NB];
+              ldv<NB>(X, srow + (ta & 0xffff) * NB);
+              ldv<NB>(Y, srow + (ta >> 16) * NB);
+              ldv<NB>(Xb, srow + (tbb & 0xffff) * NB);
+              ldv<NB>(Yb, srow + (tbb >> 16) * NB);
+  #pragma unroll
+              for (int b = 0; b < NB; b++) {
+                S1[b] = X[b] + Xb[b];
+                S3[b] = Y[b] + Yb[b];
+                Te[b] = X[b] + Yb[b];
+                To[b] = Y[b] + Xb[b];
+              }
+            }
+            if (wv & 0x300) {  // pieces 2 and 3 (z boundaries inside the segment)
+              const float p2 = (wv & 0x100) ? rdlane(P2[s], d) : 0.f;
+              const float p3 = (wv & 0x200) ? rdlane(P3[s], d) : 0.f;
+              const float a2 = odd ? p2 : 0.f, a3 = odd ? 0.f : p3;
+              const float c2 = odd ? 0.f : p2, c3 = odd ? p3 : 0.f;
+  #pragma unroll
+              for (int b = 0; b < NB; b++) {
+                acc0[s][b] = fmaf(a2, To[b], fmaf(a3, S3[b], acc0[s][b]));
+                acc1[s][b] = fmaf(c2, Te[b], fmaf(c3, S3[b], acc1[s][b]));
+              }
+            }
+          }
+          // piece 1: plane 2q (even base plane) or 2q+1 (odd); the coefficients
+          // are selected on the scalar unit, both FMAs always run
+          const float a1 = odd ? 0.f : p1, c1 = odd ? p1 : 0.f;
+  #pragma unroll
           for (int b = 0; b < NB; b++) {
-            S1[b] = X[b] + Xb[b];
-            S3[b] = Y[b] + Yb[b];
-            Te[b] = X[b] + Yb[b];
-            To[b] = Y[b] + Xb[b];
+            acc0[s][b] = fmaf(a1, S1[b], acc0[s][b]);
+            acc1[s][b] = fmaf(c1, S1[b], acc1[s][b]);
           }
-        }
-        const float p1 = rdlane(P1[s], d);
-        if (w0 & 1) {
-#pragma unroll
-          for (int b = 0; b < NB; b++) acc1[s][b] = fmaf(p1, S1[b], acc1[s][b]);
-        } else {
-#pragma unroll
-          for (int b = 0; b < NB; b++) acc0[s][b] = fmaf(p1, S1[b], acc0[s][b]);
-        }
-        if (wv & 0x100) {  // a z boundary inside the segment (zero pieces are skipped)
-          const float p2 = rdlane(P2[s], d);
-          if (w0 & 1) {
-#pragma unroll
-            for (int b = 0; b < NB; b++) acc0[s][b] = fmaf(p2, To[b], acc0[s][b]);
-          } else {
-#pragma unroll
-            for (int b = 0; b < NB; b++) acc1[s][b] = fmaf(p2, Te[b], acc1[s][b]);
-          }
-        }
-        if (wv & 0x200) {
-          const float p3 = rdlane(P3[s], d);
-          if (w0 & 1) {
-#pragma unroll
-            for (int b = 0; b < NB; b++) acc1[s][b] = fmaf(p3, S3[b], acc1[s][b]);
-          } else {
-#pragma unroll
-            for (int b = 0; b < NB; b++) acc0[s][b] = fmaf(p3, S3[b], acc0[s][b]);
-          }
-        }
-      });
+        });
+      }
     }
     // flush the wave's segments through its LDS buffer (wave-local, no barrier)
     static_for<SPW>([&](auto sc) {
