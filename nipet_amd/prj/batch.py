@@ -15,17 +15,22 @@ normalisation (hence the 14 sensitivity images) run as one batch:
 * the whole ``itr`` x 14 OSEM of all nb images is captured once as a HIP
   graph (``torch.cuda.CUDAGraph`` over the library's stream launches: no host
   synchronisation, allocation or memset inside) and runs as one graph launch;
+* replicates are stored in contiguous groups of ``GROUP`` ([group][voxel][4],
+  prompts [group][bin][837][4]): a kernel launch covers one group, so every
+  cache line it touches holds only that group's replicates;
 * the replicate sub-batch width of the FP and BP kernels (4, 2 or 1 images per
   launch) is picked by timing both kernels on the problem's own geometry and
-  batch size (``autotune``).
+  group size (``autotune``).
 """
 import numpy as np
 
 from .. import _lib
 from .osem import OsemBatch, osem_iteration_batch, osem_prepare_batch
 
-__all__ = ['autotune', 'OsemGraph', 'bootstrap_prompts', 'frame_prompts', 'bootstrap_osem',
-           'frames_osem']
+__all__ = ['GROUP', 'autotune', 'prepare_groups', 'OsemGraph', 'bootstrap_prompts',
+           'frame_prompts', 'bootstrap_osem', 'frames_osem']
+
+GROUP = 4  # replicates per contiguous group (the widest kernel instance)
 
 
 def _torch():
@@ -87,51 +92,94 @@ def autotune(state, widths=(4, 2, 1), repeats=2, probe=4):
     return res
 
 
-class OsemGraph:
-    """``niter`` batched OSEM iterations of ``state`` captured as one HIP graph
-    on a persistent image buffer [320, 320, 127, nb]; ``run`` resets the
-    image (ones, or ``img0``) and launches the graph."""
+def _per_rep(x):
+    return x is not None and hasattr(x, 'dim') and x.dim() == 3
 
-    def __init__(self, state, niter):
+
+def prepare_groups(scanner_params, prompt_groups, asng, nsng, rsng=None, ssng=None, sens=None):
+    """One ``OsemBatch`` per replicate group (``prompt_groups[i]``: CUDA uint16
+    (Naw, 837, g_i), the first group the largest). Attenuation and norm are
+    shared; randoms / scatter are shared (Naw, 837) or per replicate
+    (Naw, 837, nb) CUDA tensors, sliced by group. Sensitivity images, mask,
+    shared rho and the work buffers are common to all groups."""
+    states, off, first = [], 0, None
+    per = _per_rep(rsng) or _per_rep(ssng)
+    for p in prompt_groups:
+        g = p.shape[2]
+        if first is not None and not per:
+            st = OsemBatch(first.geom, first.subsets, p, first.rho, False, first.sens,
+                           first.mask, g, work=(first.work_sino, first.work_img))
+        else:
+            def sl(x):
+                return x[..., off:off + g].contiguous() if _per_rep(x) else x
+            st = osem_prepare_batch(
+                scanner_params, p, asng, nsng, sl(rsng), sl(ssng),
+                sens=sens if first is None else first.sens,
+                work=None if first is None else (first.work_sino, first.work_img))
+        if first is None:
+            first = st
+        states.append(st)
+        off += g
+    return states
+
+
+class OsemGraph:
+    """``niter`` OSEM iterations of every replicate group (``OsemBatch`` or a
+    list of them) captured as one HIP graph on persistent image buffers
+    ([320, 320, 127, g] per group); ``run`` resets the images (ones, or
+    ``img0`` [320, 320, 127, nb]) and launches the graph."""
+
+    def __init__(self, states, niter):
         torch = _torch()
-        if not isinstance(state, OsemBatch):
-            raise TypeError('state must be an OsemBatch')
-        self.state, self.niter = state, int(niter)
-        self.img = torch.ones((320, 320, 127, state.nb), dtype=torch.float32, device='cuda')
+        states = [states] if isinstance(states, OsemBatch) else list(states)
+        if not states or not all(isinstance(st, OsemBatch) for st in states):
+            raise TypeError('states must be OsemBatch objects')
+        self.states, self.niter = states, int(niter)
+        self.imgs = [torch.ones((320, 320, 127, st.nb), dtype=torch.float32, device='cuda')
+                     for st in states]
         _lib.check(_lib.lib().nipet_prof_enable(0), 'nipet_prof_enable')
         # load every kernel instance before capture: one subset update over a
-        # single transaxial bin (the image is reset before each run)
-        first = state.subsets[0][:1].contiguous()
-        _lib.check(_lib.lib().nipet_osem_subset_b(
-            state.geom.handle, _lib.ptr(self.img), _lib.ptr(state.prompts),
-            _lib.ptr(state.rho), int(state.rho_per_rep), _lib.ptr(state.sens[0]), 0,
-            _lib.ptr(state.mask), _lib.ptr(first), 1, state.nb, _lib.ptr(state.work_sino),
-            _lib.ptr(state.work_img), _lib.stream_handle()), 'nipet_osem_subset_b')
+        # single transaxial bin per group (the images are reset before each run)
+        for st, img in zip(states, self.imgs):
+            first = st.subsets[0][:1].contiguous()
+            _lib.check(_lib.lib().nipet_osem_subset_b(
+                st.geom.handle, _lib.ptr(img), _lib.ptr(st.prompts), _lib.ptr(st.rho),
+                int(st.rho_per_rep), _lib.ptr(st.sens[0]), 0, _lib.ptr(st.mask),
+                _lib.ptr(first), 1, st.nb, _lib.ptr(st.work_sino), _lib.ptr(st.work_img),
+                _lib.stream_handle()), 'nipet_osem_subset_b')
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
             for _ in range(self.niter):
-                osem_iteration_batch(state, self.img)
+                for st, img in zip(states, self.imgs):
+                    osem_iteration_batch(st, img)
         torch.cuda.synchronize()
 
     def run(self, img0=None):
-        if img0 is None:
-            self.img.fill_(1.0)
-        else:
-            self.img.copy_(img0)
+        off = 0
+        for img in self.imgs:
+            if img0 is None:
+                img.fill_(1.0)
+            else:
+                img.copy_(img0[..., off:off + img.shape[3]])
+            off += img.shape[3]
         self.graph.replay()
-        return self.img
+        return self.imgs[0] if len(self.imgs) == 1 else _torch().cat(self.imgs, dim=3)
 
 
-def bootstrap_prompts(words, scanner_params, nrep, seed, rep0=0, t0=0, t1=0, hist=None):
-    """Batched uint16 prompts (Naw, 837, nrep) of the bootstrap replicates
-    rep0 .. rep0+nrep-1 of the frame [t0, t1) of a device word array (one
-    list-mode pass). Returns (prompts, LmHistogram)."""
-    torch = _torch()
+def bootstrap_prompts(words, scanner_params, nrep, seed, rep0=0, t0=0, t1=0, hist=None,
+                      group=None):
+    """Batched uint16 prompts of the bootstrap replicates rep0 .. rep0+nrep-1
+    of the frame [t0, t1) of a device word array (one list-mode pass): one
+    (Naw, 837, nrep) tensor, or with ``group`` a list of (Naw, 837, g) tensors
+    of consecutive replicate groups. Returns (prompts, LmHistogram)."""
     from ..lm.mmrhist import hist_words
     h = hist if hist is not None else hist_words(words, scanner_params, t0, t1, btp=2, seed=seed,
                                                  nrep=nrep, rep0=rep0)
-    return _remgaps_b(h, scanner_params, h.nrep, 0, h.nrep), h
+    if group is None:
+        return _remgaps_b(h, scanner_params, h.nrep, 0, h.nrep), h
+    return [_remgaps_b(h, scanner_params, h.nrep, r0, min(group, h.nrep - r0))
+            for r0 in range(0, h.nrep, group)], h
 
 
 def _remgaps_b(h, scanner_params, nrep, r0, nb):
@@ -190,21 +238,26 @@ def bootstrap_osem(words, scanner_params, asng, nsng, rsng=None, ssng=None, nrep
         t0[0] = now
 
     mark(None)
-    prompts, h = bootstrap_prompts(words, scanner_params, nrep, seed, rep0, t0, t1)
+    groups, h = bootstrap_prompts(words, scanner_params, nrep, seed, rep0, t0, t1,
+                                  group=GROUP)
     del h
     mark('histogram_s')
-    st = osem_prepare_batch(scanner_params, prompts, asng, nsng, rsng, ssng, sens=sens)
+    states = prepare_groups(scanner_params, groups, asng, nsng, rsng, ssng, sens=sens)
+    st = states[0]
     mark('prepare_s')
     info = {'tune': autotune(st) if tune else None}
     mark('tune_s')
     if graph:
-        g = OsemGraph(st, itr)
+        g = OsemGraph(states, itr)
         mark('capture_s')
         img = g.run().clone()
     else:
-        img = torch.ones((320, 320, 127, st.nb), dtype=torch.float32, device='cuda')
+        imgs = [torch.ones((320, 320, 127, x.nb), dtype=torch.float32, device='cuda')
+                for x in states]
         for _ in range(itr):
-            osem_iteration_batch(st, img)
+            for x, im in zip(states, imgs):
+                osem_iteration_batch(x, im)
+        img = torch.cat(imgs, dim=3)
     mark('osem_s')
     info['sens'] = st.sens
     info['timing'] = tm
@@ -218,12 +271,17 @@ def frames_osem(words, scanner_params, frames, asng, nsng, rho_frames, itr=4, tu
     per frame (Naw, 837, nf). Returns (images [320, 320, 127, nf], info)."""
     torch = _torch()
     prompts, dtot = frame_prompts(words, scanner_params, frames)
-    st = osem_prepare_batch(scanner_params, prompts, asng, nsng, rho_frames, None, sens=sens)
-    info = {'tune': autotune(st) if tune else None, 'delayed': dtot}
+    nf = prompts.shape[2]
+    groups = [prompts[..., f0:f0 + GROUP].contiguous() for f0 in range(0, nf, GROUP)]
+    states = prepare_groups(scanner_params, groups, asng, nsng, rho_frames, None, sens=sens)
+    info = {'tune': autotune(states[0]) if tune else None, 'delayed': dtot}
     if graph:
-        img = OsemGraph(st, itr).run().clone()
+        img = OsemGraph(states, itr).run().clone()
     else:
-        img = torch.ones((320, 320, 127, st.nb), dtype=torch.float32, device='cuda')
+        imgs = [torch.ones((320, 320, 127, x.nb), dtype=torch.float32, device='cuda')
+                for x in states]
         for _ in range(itr):
-            osem_iteration_batch(st, img)
+            for x, im in zip(states, imgs):
+                osem_iteration_batch(x, im)
+        img = torch.cat(imgs, dim=3)
     return img, info

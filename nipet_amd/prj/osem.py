@@ -17,6 +17,9 @@ from ..geom import get_geom
 __all__ = ['OsemState', 'osem_prepare', 'osem_run', 'sensitivity']
 
 
+NVOX = 320 * 320 * 127
+
+
 def _torch():
     import torch
     return torch
@@ -128,19 +131,26 @@ class OsemBatch:
     (shared rho) or frames of one scan (rho per frame). Images are
     [320, 320, 127, nb] (replicate innermost)."""
 
-    def __init__(self, geom, subsets, prompts, rho, rho_per_rep, sens, mask, nb):
+    def __init__(self, geom, subsets, prompts, rho, rho_per_rep, sens, mask, nb, work=None):
         torch = _torch()
         self.geom, self.subsets, self.nb = geom, subsets, nb
         self.prompts, self.rho, self.rho_per_rep, self.sens, self.mask = (
             prompts, rho, rho_per_rep, sens, mask)
         self.nprj = max(len(s) for s in subsets)
-        self.work_sino = torch.empty((self.nprj, geom.nsn, nb), dtype=torch.float32,
-                                     device='cuda')
-        self.work_img = torch.empty((320, 320, 127, nb), dtype=torch.float32, device='cuda')
+        if work is not None:  # (work_sino, work_img) of another batch, at least this large
+            ws, wi = work
+            if ws.numel() < self.nprj * geom.nsn * nb or wi.numel() < NVOX * nb:
+                raise ValueError('shared work buffers are too small for this batch')
+            self.work_sino, self.work_img = ws, wi
+        else:
+            self.work_sino = torch.empty((self.nprj, geom.nsn, nb), dtype=torch.float32,
+                                         device='cuda')
+            self.work_img = torch.empty((320, 320, 127, nb), dtype=torch.float32,
+                                        device='cuda')
 
 
 def osem_prepare_batch(scanner_params, prompts, asng, nsng, rsng=None, ssng=None,
-                       mask_radius=29., subsets=None, mask=None, sens=None):
+                       mask_radius=29., subsets=None, mask=None, sens=None, work=None):
     """Batch state: ``prompts`` (Naw, nsn, nb) (uint16 or counts), shared
     attenuation / norm (Naw, nsn); randoms / scatter either shared (Naw, nsn) or
     per replicate (Naw, nsn, nb). ``sens`` may be passed from an earlier batch
@@ -192,7 +202,7 @@ def osem_prepare_batch(scanner_params, prompts, asng, nsng, rsng=None, ssng=None
         mask = get_cylinder(Cnt, rad=mask_radius, xo=0, yo=0, unival=1, gpu_dim=True) > 0.9
     msk = _dev(mask if isinstance(mask, torch.Tensor) else np.asarray(mask, np.uint8),
                torch.uint8)
-    return OsemBatch(geom, subs, p, rho, per_rep, sens, msk, nb)
+    return OsemBatch(geom, subs, p, rho, per_rep, sens, msk, nb, work=work)
 
 
 def osem_iteration_batch(state, img):
