@@ -146,6 +146,24 @@ int nipet_osem_subset_b(const NipetGeom *g, float *img, const uint16_t *prompts,
                         const uint8_t *mask, const int32_t *subs, int nprj, int nb,
                         float *work_sino, float *work_img, void *stream);
 
+/* Resolution modelling: separable 17-tap convolution of nb replicate-innermost
+ * device images [voxel][nb] with the host kernels krnl[3][17] (rows: z, x, y
+ * taps as psf_config builds them, prj/mmrrec.py:96-155), zero outside the
+ * image, out[i] = sum_j k[8-j] in[i+j]. Replaces d_conv (prj/src/recon.cu:
+ * 49-215). out must differ from in and tmp; tmp may be in (then overwritten). */
+int nipet_psf_conv(float *out, const float *in, float *tmp, const float *krnl, int nb,
+                   void *stream);
+
+/* nipet_osem_subset_b with resolution modelling (petprj.osem with krnl[0][0]
+ * >= 0, prj/src/recon.cu:468-500): the subset update of H x, back projection
+ * convolved by H before the division by the (convolved) sensitivity. Two work
+ * images [voxel][nb]. */
+int nipet_osem_subset_rm_b(const NipetGeom *g, float *img, const uint16_t *prompts,
+                           const float *rho, int rho_per_rep, const float *sens,
+                           int sens_per_rep, const uint8_t *mask, const int32_t *subs, int nprj,
+                           int nb, const float *krnl, float *work_sino, float *work_img,
+                           float *work_img2, void *stream);
+
 /* ---- list-mode histogramming (mmr_lmproc.hist: lm/src/lm_module.cu:174-391,
  * lmproc lm/src/lmproc.cu:12-230, gpu_hst / hst lm/src/hst.cu:37-528) ----
  * A session histograms one frame [tstart, tstop) (seconds from the first time

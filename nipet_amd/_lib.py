@@ -66,6 +66,8 @@ def lib():
             'nipet_putgaps': ([P, P, P, I, I, I, P], I),
             'nipet_remgaps_u16b': ([P, P, P, I, I, I, I, I, I, P], I),
             'nipet_tune_set': ([I, I], I),
+            'nipet_psf_conv': ([P, P, P, P, I, P], I),
+            'nipet_osem_subset_rm_b': ([P, P, P, P, I, P, I, P, P, I, I, P, P, P, P, P], I),
             'nipet_tune_get': ([I, P], I),
             'nipet_osem_subset': ([P, P, P, P, P, P, P, I, P, P, P], I),
             'nipet_fprj_b': ([P, P, P, P, I, I, I, P], I),

@@ -15,6 +15,7 @@ constexpr int64_t NVOX = (int64_t)NX * NY * NZ;
 constexpr int MAXNSN = 4096;  // largest sinogram count handled (span-1: 4084)
 constexpr int LI_MAX = 2112;  // largest Michelogram (ring pair) count
 constexpr float EPS = 1e-6f;  // OSEM division guard (prj/src/recon.cu:12)
+constexpr int PSF_R = 8;      // PSF kernel half-width (17 taps)
 
 // One transaxial segment: start t (cm from the first crystal, double so that
 // z-boundary positions along the ray keep sub-nanometre precision), length
@@ -49,6 +50,13 @@ struct Geom {
 
 void set_error(const char *fmt, ...);
 int hip_fail(hipError_t e, const char *what);
+
+// Separable 17-tap convolution of nb replicate-innermost images, kernels
+// krnl[3][17] (host) for z, x, y: in -> out, tmp a scratch image (out must
+// differ from in and tmp; tmp may be in, which is then overwritten). Launches
+// on st only (no synchronisation, no allocation).
+int psf_conv(float *out, const float *in, float *tmp, const float *krnl, int nb,
+             hipStream_t st);
 
 // Opt-in kernel timing (nipet_prof_*): HIP events recorded around launches on
 // the launch stream, resolved when read.

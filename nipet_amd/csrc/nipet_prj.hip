@@ -990,6 +990,45 @@ int nipet_osem_subset_b(const NipetGeom *g, float *img, const uint16_t *prompts,
   return NIPET_OK;
 }
 
+int nipet_osem_subset_rm_b(const NipetGeom *g, float *img, const uint16_t *prompts,
+                           const float *rho, int rho_per_rep, const float *sens,
+                           int sens_per_rep, const uint8_t *mask, const int32_t *subs, int nprj,
+                           int nb, const float *krnl, float *work_sino, float *work_img,
+                           float *work_img2, void *stream) {
+  int rc = check_geom(g, "nipet_osem_subset_rm");
+  if (rc) return rc;
+  if (!subs || nprj <= 0 || nprj > g->naw) {
+    set_error("nipet_osem_subset_rm: subs must be given with 0 < nprj <= naw");
+    return NIPET_EINVAL;
+  }
+  if ((rc = check_nprj(g, subs, nprj, nb, "nipet_osem_subset_rm"))) return rc;
+  if (!img || !prompts || !rho || !sens || !mask || !krnl || !work_sino || !work_img ||
+      !work_img2 || work_img == work_img2) {
+    set_error("nipet_osem_subset_rm: NULL or aliased buffer");
+    return NIPET_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  // resolution-modelled image H x (work_img is the convolution scratch)
+  if ((rc = psf_conv(work_img2, img, work_img, krnl, nb, st))) return rc;
+  prof_begin(PK_FPRJ, st);
+  launch_fprj<2>(g, work_sino, work_img2, subs, nprj, prompts, rho, rho_per_rep ? 1 : 0, nb, st);
+  prof_end(PK_FPRJ, st);
+  prof_begin(PK_FILL, st);
+  k_fill<<<ew_blocks(NVOX * nb), 256, 0, st>>>(work_img, 0.f, NVOX * nb);
+  prof_end(PK_FILL, st);
+  prof_begin(PK_BPRJ, st);
+  launch_bprj(g, work_img, work_sino, subs, nprj, nb, st);
+  prof_end(PK_BPRJ, st);
+  // H^T of the back projection (symmetric kernels: H), into work_img2
+  if ((rc = psf_conv(work_img2, work_img, work_img, krnl, nb, st))) return rc;
+  prof_begin(PK_OSEM_UPDATE, st);
+  k_osem_update<<<ew_blocks(NVOX * nb), 256, 0, st>>>(img, work_img2, sens, mask, NVOX, nb,
+                                                       sens_per_rep ? 1 : 0);
+  prof_end(PK_OSEM_UPDATE, st);
+  NIPET_HIP(hipGetLastError());
+  return NIPET_OK;
+}
+
 int nipet_osem_subset(const NipetGeom *g, float *img, const uint16_t *prompts, const float *rho,
                       const float *sens, const uint8_t *mask, const int32_t *subs, int nprj,
                       float *work_sino, float *work_img, void *stream) {

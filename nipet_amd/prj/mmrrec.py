@@ -48,8 +48,9 @@ def get_subsets14(n, params):
 
 
 def psf_config(psf, Cnt):
-    """Separable PSF kernel (3, 17) ordered (z, y, x); ``[0,0] < 0`` switches
-    resolution modelling off (``prj/mmrrec.py:96-155``)."""
+    """Separable PSF kernel (3, 17): rows act on z, x and y of the device image
+    (the reference's order, built from FWHM [x, y, z] in mm); ``[0,0] < 0``
+    switches resolution modelling off (``prj/mmrrec.py:96-155``)."""
     R = Cnt['RSZ_PSF_KRNL']
 
     def _config(fwhm3, check_len=True):
@@ -139,11 +140,9 @@ def osemone(datain, mumaps, hst, scanner_params, recmod=3, itr=4, fwhm=0., psf=N
     rsng = None if randsino is None else dev(randsino)
     ssng = None if (sctsino is None or recmod < 2) else dev(sctsino)
     psfkernel = psf_config(psf, Cnt)
-    if psfkernel[0, 0] >= 0:
-        raise NotImplementedError('PSF resolution modelling is not part of this path yet')
     stime = time.time()
     state = _osem.osem_prepare(scanner_params, psng, asng, nsng, rsng, ssng,
-                               mask_radius=mask_radius)
+                               mask_radius=mask_radius, psf=psfkernel)
     img = _osem.osem_run(state, itr)
     log.info('recon time: %.3g', time.time() - stime)
     im = mmrimg.convert2e7(img, Cnt)

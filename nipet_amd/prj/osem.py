@@ -35,7 +35,7 @@ def _dev(x, dtype):
 class OsemState:
     """Everything one OSEM reconstruction keeps on the device."""
 
-    def __init__(self, geom, subsets, prompts, rho, an, sens, mask):
+    def __init__(self, geom, subsets, prompts, rho, an, sens, mask, krnl=None):
         torch = _torch()
         self.geom = geom
         self.subsets = subsets
@@ -43,6 +43,9 @@ class OsemState:
         self.prompts, self.rho, self.an, self.sens, self.mask = prompts, rho, an, sens, mask
         self.work_sino = torch.empty((self.nprj, geom.nsn), dtype=torch.float32, device='cuda')
         self.work_img = torch.empty((320, 320, 127), dtype=torch.float32, device='cuda')
+        # resolution modelling: host kernel (3, 17) and a second work image
+        self.krnl = krnl
+        self.work_img2 = None if krnl is None else torch.empty_like(self.work_img)
 
 
 def sensitivity(geom, an, subs):
@@ -55,11 +58,40 @@ def sensitivity(geom, an, subs):
     return out
 
 
+def _psf_kernel(psf):
+    """Host float32 (3, 17) kernel, or None when resolution modelling is off
+    (``psf`` None or ``psf[0, 0] < 0``, the reference's switch)."""
+    if psf is None:
+        return None
+    k = np.ascontiguousarray(np.asarray(psf, dtype=np.float32))
+    if k.shape != (3, 17):
+        raise ValueError('psf must be a (3, 17) separable kernel (psf_config)')
+    return None if k[0, 0] < 0 else k
+
+
+def psf_conv(img, krnl, out=None, tmp=None):
+    """Resolution-model convolution of device image(s) [320, 320, 127(, nb)]
+    with a (3, 17) kernel (``nipet_psf_conv``); returns ``out``."""
+    torch = _torch()
+    k = _psf_kernel(krnl)
+    if k is None:
+        raise ValueError('psf kernel switched off')
+    nb = 1 if img.dim() == 3 else img.shape[3]
+    out = torch.empty_like(img) if out is None else out
+    tmp = torch.empty_like(img) if tmp is None else tmp
+    _lib.check(_lib.lib().nipet_psf_conv(_lib.ptr(out), _lib.ptr(img), _lib.ptr(tmp),
+                                         k.ctypes.data, nb, _lib.stream_handle()),
+               'nipet_psf_conv')
+    return out
+
+
 def osem_prepare(scanner_params, psng, asng, nsng, rsng=None, ssng=None, mask_radius=29.,
-                 subsets=None, mask=None):
+                 subsets=None, mask=None, psf=None):
     """Upload the data and compute rho = (r+s)/(a n), a n and the per-subset
     sensitivity images. Sinograms are device-shaped (Naw, nsn); prompts are
-    converted to uint16 as in ``petprj.osem`` (``prj/src/prj_module.cu:649-651``)."""
+    converted to uint16 as in ``petprj.osem`` (``prj/src/prj_module.cu:649-651``).
+    ``psf``: (3, 17) kernel from ``psf_config`` (resolution modelling: the
+    sensitivity images are convolved once, ``prj/src/recon.cu:441-445``)."""
     torch = _torch()
     from ..img.mmrimg import get_cylinder
     from .mmrrec import get_subsets14
@@ -92,12 +124,18 @@ def osem_prepare(scanner_params, psng, asng, nsng, rsng=None, ssng=None, mask_ra
         subsets = [get_subsets14(k, scanner_params)[0] for k in range(14)]
     subs = [x.to('cuda', torch.int32).contiguous() if isinstance(x, torch.Tensor) else
             _dev(np.asarray(x, np.int32), torch.int32) for x in subsets]
+    krnl = _psf_kernel(psf)
     sens = torch.stack([sensitivity(geom, an, x) for x in subs])
+    if krnl is not None:
+        tmp = torch.empty((320, 320, 127), dtype=torch.float32, device='cuda')
+        for k in range(len(subs)):
+            tmp.copy_(sens[k])
+            psf_conv(tmp, krnl, out=sens[k], tmp=tmp)
     if mask is None:
         mask = get_cylinder(Cnt, rad=mask_radius, xo=0, yo=0, unival=1, gpu_dim=True) > 0.9
     msk = _dev(mask if isinstance(mask, torch.Tensor) else np.asarray(mask, np.uint8),
                torch.uint8)
-    return OsemState(geom, subs, prompts, rho, an, sens, msk)
+    return OsemState(geom, subs, prompts, rho, an, sens, msk, krnl)
 
 
 def osem_iteration(state, img):
@@ -105,6 +143,13 @@ def osem_iteration(state, img):
     st = _lib.stream_handle()
     L = _lib.lib()
     for k, subs in enumerate(state.subsets):
+        if state.krnl is not None:
+            _lib.check(L.nipet_osem_subset_rm_b(
+                state.geom.handle, _lib.ptr(img), _lib.ptr(state.prompts), _lib.ptr(state.rho), 0,
+                _lib.ptr(state.sens[k]), 0, _lib.ptr(state.mask), _lib.ptr(subs), len(subs), 1,
+                state.krnl.ctypes.data, _lib.ptr(state.work_sino), _lib.ptr(state.work_img),
+                _lib.ptr(state.work_img2), st), 'nipet_osem_subset_rm_b')
+            continue
         _lib.check(L.nipet_osem_subset(state.geom.handle, _lib.ptr(img), _lib.ptr(state.prompts),
                                        _lib.ptr(state.rho), _lib.ptr(state.sens[k]),
                                        _lib.ptr(state.mask), _lib.ptr(subs), len(subs),
@@ -131,9 +176,11 @@ class OsemBatch:
     (shared rho) or frames of one scan (rho per frame). Images are
     [320, 320, 127, nb] (replicate innermost)."""
 
-    def __init__(self, geom, subsets, prompts, rho, rho_per_rep, sens, mask, nb, work=None):
+    def __init__(self, geom, subsets, prompts, rho, rho_per_rep, sens, mask, nb, work=None,
+                 krnl=None):
         torch = _torch()
         self.geom, self.subsets, self.nb = geom, subsets, nb
+        self.krnl = krnl
         self.prompts, self.rho, self.rho_per_rep, self.sens, self.mask = (
             prompts, rho, rho_per_rep, sens, mask)
         self.nprj = max(len(s) for s in subsets)
@@ -147,10 +194,17 @@ class OsemBatch:
                                          device='cuda')
             self.work_img = torch.empty((320, 320, 127, nb), dtype=torch.float32,
                                         device='cuda')
+        self.work_img2 = None
+        if krnl is not None:  # resolution modelling: a second work image
+            self.work_img2 = (work[2] if work is not None and len(work) > 2 and
+                              work[2].numel() >= NVOX * nb else
+                              torch.empty((320, 320, 127, nb), dtype=torch.float32,
+                                          device='cuda'))
 
 
 def osem_prepare_batch(scanner_params, prompts, asng, nsng, rsng=None, ssng=None,
-                       mask_radius=29., subsets=None, mask=None, sens=None, work=None):
+                       mask_radius=29., subsets=None, mask=None, sens=None, work=None,
+                       psf=None):
     """Batch state: ``prompts`` (Naw, nsn, nb) (uint16 or counts), shared
     attenuation / norm (Naw, nsn); randoms / scatter either shared (Naw, nsn) or
     per replicate (Naw, nsn, nb). ``sens`` may be passed from an earlier batch
@@ -196,19 +250,33 @@ def osem_prepare_batch(scanner_params, prompts, asng, nsng, rsng=None, ssng=None
         subsets = [get_subsets14(k, scanner_params)[0] for k in range(14)]
     subs = [x.to('cuda', torch.int32).contiguous() if isinstance(x, torch.Tensor) else
             _dev(np.asarray(x, np.int32), torch.int32) for x in subsets]
+    krnl = _psf_kernel(psf)
     if sens is None:
         sens = torch.stack([sensitivity(geom, an, x) for x in subs])
+        if krnl is not None:
+            tmp = torch.empty((320, 320, 127), dtype=torch.float32, device='cuda')
+            for k in range(len(subs)):
+                tmp.copy_(sens[k])
+                psf_conv(tmp, krnl, out=sens[k], tmp=tmp)
     if mask is None:
         mask = get_cylinder(Cnt, rad=mask_radius, xo=0, yo=0, unival=1, gpu_dim=True) > 0.9
     msk = _dev(mask if isinstance(mask, torch.Tensor) else np.asarray(mask, np.uint8),
                torch.uint8)
-    return OsemBatch(geom, subs, p, rho, per_rep, sens, msk, nb, work=work)
+    return OsemBatch(geom, subs, p, rho, per_rep, sens, msk, nb, work=work, krnl=krnl)
 
 
 def osem_iteration_batch(state, img):
     st = _lib.stream_handle()
     L = _lib.lib()
     for k, subs in enumerate(state.subsets):
+        if state.krnl is not None:
+            _lib.check(L.nipet_osem_subset_rm_b(
+                state.geom.handle, _lib.ptr(img), _lib.ptr(state.prompts), _lib.ptr(state.rho),
+                int(state.rho_per_rep), _lib.ptr(state.sens[k]), 0, _lib.ptr(state.mask),
+                _lib.ptr(subs), len(subs), state.nb, state.krnl.ctypes.data,
+                _lib.ptr(state.work_sino), _lib.ptr(state.work_img), _lib.ptr(state.work_img2),
+                st), 'nipet_osem_subset_rm_b')
+            continue
         _lib.check(L.nipet_osem_subset_b(
             state.geom.handle, _lib.ptr(img), _lib.ptr(state.prompts), _lib.ptr(state.rho),
             int(state.rho_per_rep), _lib.ptr(state.sens[k]), 0, _lib.ptr(state.mask),

@@ -120,19 +120,47 @@ def bprj(params, sino, rays, nthreads=0):
     return img
 
 
-def osem(params, prompts, rho, sens, mask, subsets, img0, niter, eps=1e-6):
+def psf_conv(img, krnl):
+    """FP64 separable convolution of a device-layout image (320 y, 320 x,
+    127 z) with krnl (3, 17) = taps for z, x, y, following d_conv
+    (prj/src/recon.cu:49-215): out[i] = sum_{j=-8..8} k[8-j] in[i+j], zero
+    outside the image, applied along z, then x, then y."""
+    x = np.asarray(img, dtype=np.float64)
+    k = np.asarray(krnl, dtype=np.float64)
+    R = (k.shape[1] - 1) // 2
+    for axis, taps in ((2, k[0]), (1, k[1]), (0, k[2])):
+        n = x.shape[axis]
+        pad = [(0, 0)] * 3
+        pad[axis] = (R, R)
+        xp = np.pad(x, pad)
+        out = np.zeros_like(x)
+        for j in range(-R, R + 1):
+            sl = [slice(None)] * 3
+            sl[axis] = slice(R + j, R + j + n)
+            out += taps[R - j] * xp[tuple(sl)]
+        x = out
+    return x
+
+
+def osem(params, prompts, rho, sens, mask, subsets, img0, niter, eps=1e-6, krnl=None):
     """FP64 OSEM following prj/src/recon.cu:459-497: per subset
     e = FP(x) + rho; e = y / e (0 where |e| < eps); b = BP(e);
     b = b / sens (0 where |sens| < eps); x = mask ? x*b : 0.
+    With ``krnl`` (resolution modelling, prj/src/recon.cu:468-495) the
+    projected image is H x and the back projection is convolved by H (the
+    caller passes sensitivity images already convolved).
     prompts/rho: (Naw, nsn) indexed by transaxial bin; sens: (nsub, 320,320,127);
     subsets: list of int arrays of transaxial bins."""
     x = np.array(img0, dtype=np.float64)
     for _ in range(niter):
         for s, rows in enumerate(subsets):
-            e = fprj(params, x, rows) + rho[rows]
+            xf = x if krnl is None else psf_conv(x, krnl)
+            e = fprj(params, xf, rows) + rho[rows]
             y = prompts[rows].astype(np.float64)
             ratio = np.where(np.abs(e) < eps, 0.0, y / np.where(np.abs(e) < eps, 1.0, e))
             b = bprj(params, ratio, rows)
+            if krnl is not None:
+                b = psf_conv(b, krnl)
             sv = sens[s]
             b = np.where(np.abs(sv) < eps, 0.0, b / np.where(np.abs(sv) < eps, 1.0, sv))
             x = np.where(mask, x * b, 0.0)
