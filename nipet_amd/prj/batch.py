@@ -96,26 +96,28 @@ def _per_rep(x):
     return x is not None and hasattr(x, 'dim') and x.dim() == 3
 
 
-def prepare_groups(scanner_params, prompt_groups, asng, nsng, rsng=None, ssng=None, sens=None):
+def prepare_groups(scanner_params, prompt_groups, asng, nsng, rsng=None, ssng=None, sens=None,
+                   psf=None):
     """One ``OsemBatch`` per replicate group (``prompt_groups[i]``: CUDA uint16
     (Naw, 837, g_i), the first group the largest). Attenuation and norm are
     shared; randoms / scatter are shared (Naw, 837) or per replicate
     (Naw, 837, nb) CUDA tensors, sliced by group. Sensitivity images, mask,
-    shared rho and the work buffers are common to all groups."""
+    shared rho and the work buffers are common to all groups. ``psf``: (3, 17)
+    resolution-modelling kernel (psf_config) or None."""
     states, off, first = [], 0, None
     per = _per_rep(rsng) or _per_rep(ssng)
     for p in prompt_groups:
         g = p.shape[2]
+        work = None if first is None else (first.work_sino, first.work_img, first.work_img2)
         if first is not None and not per:
             st = OsemBatch(first.geom, first.subsets, p, first.rho, False, first.sens,
-                           first.mask, g, work=(first.work_sino, first.work_img))
+                           first.mask, g, work=work, krnl=first.krnl)
         else:
             def sl(x):
                 return x[..., off:off + g].contiguous() if _per_rep(x) else x
             st = osem_prepare_batch(
                 scanner_params, p, asng, nsng, sl(rsng), sl(ssng),
-                sens=sens if first is None else first.sens,
-                work=None if first is None else (first.work_sino, first.work_img))
+                sens=sens if first is None else first.sens, work=work, psf=psf)
         if first is None:
             first = st
         states.append(st)
@@ -222,7 +224,7 @@ def frame_prompts(words, scanner_params, frames):
 
 
 def bootstrap_osem(words, scanner_params, asng, nsng, rsng=None, ssng=None, nrep=64, seed=1,
-                   rep0=0, itr=4, t0=0, t1=0, tune=True, graph=True, sens=None):
+                   rep0=0, itr=4, t0=0, t1=0, tune=True, graph=True, sens=None, psf=None):
     """OSEM (``itr`` x 14) of bootstrap replicates rep0 .. rep0+nrep-1 of a
     device list-mode word array, batched; corrections (Naw, 837) shared by all
     replicates. Returns (images [320, 320, 127, nrep], info)."""
@@ -242,7 +244,7 @@ def bootstrap_osem(words, scanner_params, asng, nsng, rsng=None, ssng=None, nrep
                                   group=GROUP)
     del h
     mark('histogram_s')
-    states = prepare_groups(scanner_params, groups, asng, nsng, rsng, ssng, sens=sens)
+    states = prepare_groups(scanner_params, groups, asng, nsng, rsng, ssng, sens=sens, psf=psf)
     st = states[0]
     mark('prepare_s')
     info = {'tune': autotune(st) if tune else None}
@@ -265,7 +267,7 @@ def bootstrap_osem(words, scanner_params, asng, nsng, rsng=None, ssng=None, nrep
 
 
 def frames_osem(words, scanner_params, frames, asng, nsng, rho_frames, itr=4, tune=True,
-                graph=True, sens=None):
+                graph=True, sens=None, psf=None):
     """OSEM of dynamic frames [(t0, t1), ...] of a device word array, batched;
     attenuation / norm (Naw, 837) shared, ``rho_frames`` = randoms + scatter
     per frame (Naw, 837, nf). Returns (images [320, 320, 127, nf], info)."""
@@ -273,7 +275,8 @@ def frames_osem(words, scanner_params, frames, asng, nsng, rho_frames, itr=4, tu
     prompts, dtot = frame_prompts(words, scanner_params, frames)
     nf = prompts.shape[2]
     groups = [prompts[..., f0:f0 + GROUP].contiguous() for f0 in range(0, nf, GROUP)]
-    states = prepare_groups(scanner_params, groups, asng, nsng, rho_frames, None, sens=sens)
+    states = prepare_groups(scanner_params, groups, asng, nsng, rho_frames, None, sens=sens,
+                            psf=psf)
     info = {'tune': autotune(states[0]) if tune else None, 'delayed': dtot}
     if graph:
         img = OsemGraph(states, itr).run().clone()

@@ -184,8 +184,8 @@ class OsemBatch:
         self.prompts, self.rho, self.rho_per_rep, self.sens, self.mask = (
             prompts, rho, rho_per_rep, sens, mask)
         self.nprj = max(len(s) for s in subsets)
-        if work is not None:  # (work_sino, work_img) of another batch, at least this large
-            ws, wi = work
+        if work is not None:  # (work_sino, work_img[, work_img2]) of another batch
+            ws, wi = work[0], work[1]
             if ws.numel() < self.nprj * geom.nsn * nb or wi.numel() < NVOX * nb:
                 raise ValueError('shared work buffers are too small for this batch')
             self.work_sino, self.work_img = ws, wi
@@ -197,7 +197,7 @@ class OsemBatch:
         self.work_img2 = None
         if krnl is not None:  # resolution modelling: a second work image
             self.work_img2 = (work[2] if work is not None and len(work) > 2 and
-                              work[2].numel() >= NVOX * nb else
+                              work[2] is not None and work[2].numel() >= NVOX * nb else
                               torch.empty((320, 320, 127, nb), dtype=torch.float32,
                                           device='cuda'))
 

@@ -116,3 +116,20 @@ def test_frames_batch_equals_sequential(torch, params, data):
         rel = _rel(torch, imgs[..., f], ref)
         print(f'frame {frames[f]}: rel L2 vs sequential {rel:.3g}')
         assert rel <= 1e-5
+
+
+def test_bootstrap_psf_batch_equals_sequential(torch, params, data):
+    """Resolution modelling in the grouped batch equals sequential plain runs."""
+    from nipet_amd.lm.mmrhist import hist_words
+    from nipet_amd.prj.batch import _remgaps_b, bootstrap_osem
+    from nipet_amd.prj.mmrrec import psf_config
+    from nipet_amd.prj.osem import osem_prepare, osem_run
+    k = psf_config(4.0, params['Cnt'])
+    imgs, _ = bootstrap_osem(data['words'], params, data['attn'], data['norm'], data['rs'],
+                             nrep=5, seed=31, itr=ITR, psf=k)
+    for r in (1, 4):
+        h = hist_words(data['words'], params, btp=2, seed=31, nrep=1, rep0=r)
+        p = _remgaps_b(h, params, 1, 0, 1)[..., 0].contiguous()
+        st = osem_prepare(params, p, data['attn'], data['norm'], data['rs'], psf=k)
+        ref = osem_run(st, ITR, check_counts=False)
+        assert _rel(torch, imgs[..., r], ref) <= 1e-5
