@@ -213,6 +213,36 @@ int nipet_lm_status(NipetLm *h, int64_t *words_done, unsigned int *bad_words,
                     unsigned int *singles_overflow);
 void nipet_lm_destroy(NipetLm *h);
 
+/* ---- randoms from delayed fan-sums (mmr_lmproc.rand: lm/src/lm_module.cu:404-545,
+ * gpu_randoms lm/src/rnd.cu:180-420; Python rand/randoms lm/mmrhist.py:188-221) ----
+ * Crystal singles s[c][r] (448 active crystals x 64 rings) solve
+ * fansum = s * (F s), F summing over the crystals in coincidence (cij) and the
+ * rings within MRD, by 10 (niter) damped fixed-point steps from sqrt(f / F 1);
+ * the randoms sinogram of LOR (c0, r0; c1, r1) is s[c0][r0] s[c1][r1], span-11
+ * sinograms sum their span-1 members. */
+typedef struct NipetRnd NipetRnd;
+
+typedef struct {
+  int naw;                  /* active bins (txLUT['Naw'], 68516) */
+  int nsn;                  /* output sinograms: 4084 (span 1) or 837 (span 11) */
+  int nsn1;                 /* 4084 */
+  int mrd;                  /* maximum ring difference (Cnt['MRD'], 60) */
+  const int8_t *cij;        /* host [448][448] (txLUT['cij']) */
+  const int16_t *crsri;     /* host [504] (txLUT['crsri']: reduced crystal, -1 in gaps) */
+  const int16_t *s2cr;      /* host [naw][2] (txLUT['s2cr']) */
+  const int16_t *sn1_rno;   /* host [4084][2] (axLUT['sn1_rno']) */
+  const int16_t *sn1_sn11;  /* host [4084] (axLUT['sn1_sn11']), required for span 11 */
+} NipetRndDesc;
+
+int nipet_rnd_create(const NipetRndDesc *desc, NipetRnd **out);
+/* fansums: device [64][504] delayed fan-sums (NipetLmOut.fan). Outputs
+ * (device, either may be NULL): rsino [naw][nsn] randoms sinogram in the
+ * device layout (Siemens layout through nipet_putgaps), cmap [504][64]
+ * singles map with zero gap crystals. Asynchronous on `stream`. */
+int nipet_rnd_estimate(NipetRnd *h, const uint32_t *fansums, int niter, float *rsino,
+                       float *cmap, void *stream);
+void nipet_rnd_destroy(NipetRnd *h);
+
 /* Philox4x32-10 on the device for one counter/key (host arrays), for
  * known-answer checks of the bootstrap generator. Synchronous. */
 int nipet_philox4x32(const uint32_t *ctr, const uint32_t *key, uint32_t *out);

@@ -35,6 +35,12 @@ class NipetLmOut(ct.Structure):
                 ('psino', 'dsino', 'ssr', 'phc', 'dhc', 'zR', 'zM', 'pvs', 'fan', 'bck')]
 
 
+class NipetRndDesc(ct.Structure):
+    _fields_ = [('naw', ct.c_int), ('nsn', ct.c_int), ('nsn1', ct.c_int), ('mrd', ct.c_int),
+                ('cij', ct.c_void_p), ('crsri', ct.c_void_p), ('s2cr', ct.c_void_p),
+                ('sn1_rno', ct.c_void_p), ('sn1_sn11', ct.c_void_p)]
+
+
 class NipetGeomDesc(ct.Structure):
     _fields_ = [
         ('crs', ct.c_void_p), ('ncrs', ct.c_int), ('s2c', ct.c_void_p), ('naw', ct.c_int),
@@ -78,6 +84,9 @@ def lib():
             'nipet_lm_status': ([P, P, P, P], I),
             'nipet_lm_finish': ([P, ct.POINTER(NipetLmOut), P], I),
             'nipet_lm_destroy': ([P], None),
+            'nipet_rnd_create': ([ct.POINTER(NipetRndDesc), ct.POINTER(P)], I),
+            'nipet_rnd_estimate': ([P, P, I, P, P, P], I),
+            'nipet_rnd_destroy': ([P], None),
             'nipet_philox4x32': ([P, P, P], I),
             'nipet_prof_enable': ([I], I),
             'nipet_prof_reset': ([], I),

@@ -9,6 +9,9 @@
   ``Cnt['BTPRT']``) from a reproducible counter-based generator.
 * ``LmHistogram`` -- the device session behind both: words are handed over in
   chunks (device tensors), the time-tag state carries across chunks.
+* ``randoms`` / ``rand`` -- ``lm/mmrhist.py:188-221``: randoms sinogram and
+  crystal singles map estimated from the delayed fan-sums
+  (``Randoms``, the device session holding the crystal LUTs).
 
 The file is streamed from a memory map through two pinned host buffers: the
 copy of chunk k+1 runs on a copy stream while chunk k is histogrammed.
@@ -23,7 +26,8 @@ from .. import _lib
 
 log = logging.getLogger(__name__)
 
-__all__ = ['lminfo', 'mmrhist', 'hist', 'LmHistogram', 'hist_words']
+__all__ = ['lminfo', 'mmrhist', 'hist', 'LmHistogram', 'hist_words', 'Randoms', 'randoms',
+           'rand']
 
 CHUNK_WORDS = 1 << 26      # 256 MB of words per transfer
 
@@ -266,3 +270,83 @@ def mmrhist(datain, scanner_params, t0=0, t1=0, outpath='', frms=None, use_store
     Cnt, txLUT, axLUT = scanner_params['Cnt'], scanner_params['txLUT'], scanner_params['axLUT']
     return hist(datain, txLUT, axLUT, Cnt, frms=frms, use_stored=use_stored, store=store,
                 outpath=outpath, t0=t0, t1=t1, cmass_sig=cmass_sig)
+
+
+# ------------------------------------------------------------------------------
+# randoms from the delayed fan-sums
+# ------------------------------------------------------------------------------
+
+RND_NITER = 10     # fixed-point steps of gpu_randoms (lm/src/rnd.cu:339)
+
+
+class Randoms:
+    """Device session of the randoms estimate from delayed fan-sums
+    (``mmr_lmproc.rand``, ``lm/src/rnd.cu:180-420``). The crystal
+    coincidence matrix and sinogram LUTs are uploaded once; ``__call__`` takes
+    device or host fan-sums (64, 504) and returns device tensors:
+    the randoms sinogram in the device layout (Naw, nsinos) and the crystal
+    singles map (504, 64), zero in the gap crystals."""
+
+    def __init__(self, scanner_params, span=None):
+        Cnt, tx, ax = scanner_params['Cnt'], scanner_params['txLUT'], scanner_params['axLUT']
+        self.span = Cnt['SPN'] if span is None else span
+        if self.span not in (1, 11):
+            raise ValueError(f'randoms: span {self.span} is not supported (1 or 11)')
+        self.nsinos = Cnt['NSN11'] if self.span == 11 else Cnt['NSN1']
+        self.naw = int(tx['Naw'])
+        self._keep = [np.ascontiguousarray(tx['cij'], np.int8),
+                      np.ascontiguousarray(tx['crsri'], np.int16),
+                      np.ascontiguousarray(tx['s2cr'], np.int16),
+                      np.ascontiguousarray(ax['sn1_rno'], np.int16),
+                      np.ascontiguousarray(ax['sn1_sn11'], np.int16)]
+        d = _lib.NipetRndDesc(self.naw, self.nsinos, Cnt['NSN1'], Cnt['MRD'],
+                              *[k.ctypes.data for k in self._keep])
+        h = ct.c_void_p()
+        _lib.check(_lib.lib().nipet_rnd_create(ct.byref(d), ct.byref(h)), 'nipet_rnd_create')
+        self.handle = h
+
+    def __del__(self):
+        h = getattr(self, 'handle', None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib._lib.nipet_rnd_destroy(h)
+            self.handle = None
+
+    def __call__(self, fansums, niter=RND_NITER, rsino=None, cmap=None):
+        torch = _torch()
+        f = fansums if isinstance(fansums, torch.Tensor) else torch.from_numpy(
+            np.ascontiguousarray(np.asarray(fansums).astype(np.uint32).view(np.int32)))
+        if tuple(f.shape) != (64, 504) or f.element_size() != 4:
+            raise ValueError('fan-sums must be a (64, 504) array of 32-bit counts')
+        f = f.to('cuda').contiguous()
+        if rsino is None:
+            rsino = torch.empty((self.naw, self.nsinos), dtype=torch.float32, device='cuda')
+        if cmap is None:
+            cmap = torch.empty((504, 64), dtype=torch.float32, device='cuda')
+        if (tuple(rsino.shape) != (self.naw, self.nsinos) or rsino.dtype != torch.float32
+                or not rsino.is_contiguous() or tuple(cmap.shape) != (504, 64)
+                or cmap.dtype != torch.float32 or not cmap.is_contiguous()):
+            raise ValueError('randoms: bad output tensors')
+        _lib.check(_lib.lib().nipet_rnd_estimate(self.handle, f.data_ptr(), int(niter),
+                                                 rsino.data_ptr(), cmap.data_ptr(),
+                                                 _lib.stream_handle()), 'nipet_rnd_estimate')
+        return rsino, cmap
+
+
+def rand(fansums, txLUT, axLUT, Cnt):
+    """Randoms sinogram (nsinos, 252, 344) with zero gaps and crystal singles
+    map (504, 64) as float32 host arrays (``lm/mmrhist.py:210-221``)."""
+    from ..mmraux import putgaps
+    r = Randoms({'Cnt': Cnt, 'txLUT': txLUT, 'axLUT': axLUT})
+    rs, cmap = r(fansums)
+    return putgaps(rs, txLUT, Cnt).cpu().numpy(), cmap.cpu().numpy()
+
+
+def randoms(hst, scanner_params, gpu_dim=False):
+    """Estimated randoms from the delayed fan-sums of a histogram
+    (``lm/mmrhist.py:188-207``): (randoms sinogram, singles map); with
+    ``gpu_dim`` the sinogram is in the device layout (Naw, nsinos)."""
+    Cnt, txLUT, axLUT = scanner_params['Cnt'], scanner_params['txLUT'], scanner_params['axLUT']
+    if gpu_dim:
+        rs, cmap = Randoms(scanner_params)(hst['fansums'])
+        return rs.cpu().numpy(), cmap.cpu().numpy()
+    return rand(hst['fansums'], txLUT, axLUT, Cnt)

@@ -5,8 +5,9 @@
   plus 4 of the remaining angles), returned as transaxial-bin (aw) indices in
   (radial bin, angle) order.
 * ``psf_config``    -- ``prj/mmrrec.py:96-155``.
-* ``osemone``       -- ``prj/mmrrec.py:158-470`` (the corrections are given as
-  arrays: this path has no scanner-file readers, randoms or scatter models).
+* ``osemone``       -- ``prj/mmrrec.py:158-470`` (corrections are arrays; the
+  randoms come from the delayed fan-sums when no randoms sinogram is given;
+  this path has no scanner-file readers or scatter model).
 """
 import logging
 import time
@@ -107,7 +108,9 @@ def osemone(datain, mumaps, hst, scanner_params, recmod=3, itr=4, fwhm=0., psf=N
     (68516,837)). Corrections are arrays in either shape: ``attnsino``
     (attenuation factors; computed from ``mumaps`` by the attenuation forward
     projector when None and recmod>0), ``normsino`` (normalisation factors,
-    ones when None), ``randsino`` and ``sctsino`` (additive; zero when None).
+    ones when None), ``randsino`` (additive; estimated from ``hst['fansums']``
+    when None, as ``prj/mmrrec.py:244-250``, zero without fan-sums) and
+    ``sctsino`` (additive; zero when None).
     Returns a namedtuple ``(im, fpet, imsmo, fsmo, affine)`` with the E7 image.
     """
     from collections import namedtuple
@@ -137,7 +140,13 @@ def osemone(datain, mumaps, hst, scanner_params, recmod=3, itr=4, fwhm=0., psf=N
         asng = mmrprj.frwd_prj(mus, scanner_params, dev_out=True, attenuation=True)
         asng = np.asarray(asng.cpu() if hasattr(asng, 'cpu') else asng)
     nsng = np.ones(psng.shape, np.float32) if normsino is None else dev(normsino)
-    rsng = None if randsino is None else dev(randsino)
+    if randsino is not None:
+        rsng = dev(randsino)
+    elif hst.get('fansums') is not None:
+        from ..lm.mmrhist import Randoms
+        rsng, _ = Randoms(scanner_params)(hst['fansums'])
+    else:
+        rsng = None
     ssng = None if (sctsino is None or recmod < 2) else dev(sctsino)
     psfkernel = psf_config(psf, Cnt)
     stime = time.time()

@@ -232,3 +232,107 @@ def lm_hist(words, params, toff, nitag, tstart=0, tstop=None, span=11, btp=0, bt
     assert n >= 0, 'bad list-mode address'
     out['counted'] = n
     return out
+
+
+# ---- randoms from delayed fan-sums ----
+
+RND_CFOR = 20     # crystals per thread row in the reference's fan-sum kernel (include/def.h:53)
+RND_ROWS = 16     # thread rows (lm/src/rnd.cu:306-308): 320 crystals walked per fan
+
+
+def rnd_ranges(cij, mrd, nrng=64):
+    """Crystal and ring ranges in coincidence, as gpu_randoms builds them
+    (lm/src/rnd.cu:205-251): crange[0..3][c] = first and last crystal of c's
+    circular run of ones in cij, wrap flag (first > last), run length;
+    rrange[0..1][r] = first and last ring with |r' - r| <= mrd."""
+    cij = np.asarray(cij)
+    n = cij.shape[0]
+    crange = np.zeros((4, n), np.int64)
+    for c1 in range(n):
+        prv = cij[c1, n - 1]
+        for c2 in range(n):
+            v = cij[c1, c2]
+            if v > prv:
+                crange[0, c1] = c2
+            if v < prv:
+                crange[1, c1] = c2 - 1 + n * (c2 == 0)
+            prv = v
+        crange[2, c1] = int(crange[0, c1] - crange[1, c1] > 0)
+        crange[3, c1] = int(cij[c1].sum())
+    r = np.arange(nrng)
+    rrange = np.stack([np.maximum(r - mrd, 0), np.minimum(r + mrd, nrng - 1)])
+    return crange, rrange
+
+
+def rnd_fan_matrices(cij, mrd, nrng=64):
+    """The fan-sum operator of the reference's kernel ``rnd`` (lm/src/rnd.cu:113-166)
+    as two 0/1 matrices: (F s)[c][r] = sum_{c', r'} C[c, c'] R[r, r'] s[c'][r'].
+    C[c] marks the crystals the kernel visits for vertex crystal c: start + k,
+    k < 320, up to the run's end (wrapping modulo the crystal count)."""
+    crange, rrange = rnd_ranges(cij, mrd, nrng)
+    n = crange.shape[1]
+    C = np.zeros((n, n), np.float64)
+    k = np.arange(RND_CFOR * RND_ROWS)
+    for c in range(n):
+        ic = crange[0, c] + k
+        if crange[2, c] == 0:
+            ic = ic[ic <= crange[1, c]]
+        else:
+            ic = ic[ic <= crange[1, c] + n]
+            ic = ic - n * (ic >= n)
+        C[c, ic] = 1.0
+    r = np.arange(nrng)
+    R = ((r[None, :] >= rrange[0][:, None]) & (r[None, :] <= rrange[1][:, None])).astype(np.float64)
+    return C, R
+
+
+def randoms(fansums, params, niter=10, span=None, guard=True, bins=None, sino=True):
+    """FP64 randoms estimate from delayed fan-sums (gpu_randoms,
+    lm/src/rnd.cu:180-420): f = fan-sums of the active crystals [c][r];
+    s = sqrt(f / F 1); niter times s = s/2 + (f / F s)/2; randoms sinogram
+    s[c0][r0] s[c1][r1] per span-1 LOR (sgl2sino, lm/src/rnd.cu:82-111),
+    summed into span-11 sinograms through sn1_sn11.
+    ``guard``: 0 where F s = 0 (the reference divides, giving NaN for 0/0).
+    ``bins``: transaxial bins (rows) of the sinogram to form (all when None).
+    Returns (rsino [bins][nsn] device layout or None, cmap [504][64], s [448][64])."""
+    Cnt, tx, ax = params['Cnt'], params['txLUT'], params['axLUT']
+    span = Cnt['SPN'] if span is None else span
+    fan = np.asarray(fansums, np.float64).reshape(Cnt['NRNG'], Cnt['NCRS'])
+    crsri = np.asarray(tx['crsri'], np.int64)
+    live = crsri >= 0
+    f = np.zeros((Cnt['NCRSR'], Cnt['NRNG']))
+    f[crsri[live]] = fan[:, live].T
+    C, R = rnd_fan_matrices(tx['cij'], Cnt['MRD'], Cnt['NRNG'])
+
+    def F(s):
+        return C @ s @ R.T
+
+    def div(a, b):
+        if not guard:
+            with np.errstate(divide='ignore', invalid='ignore'):
+                return a / b
+        return np.where(b > 0, a / np.where(b > 0, b, 1.0), 0.0)
+
+    s = np.sqrt(div(f, F(np.ones_like(f))))
+    for _ in range(niter):
+        s = 0.5 * s + 0.5 * div(f, F(s))
+    cmap = np.zeros((Cnt['NCRS'], Cnt['NRNG']))
+    cmap[live] = s[crsri[live]]
+    if not sino:
+        return None, cmap, s
+    s2cr = np.asarray(tx['s2cr'], np.int64)
+    if bins is not None:
+        s2cr = s2cr[np.asarray(bins, np.int64)]
+    rno = np.asarray(ax['sn1_rno'], np.int64)
+    S0 = np.ascontiguousarray(s[s2cr[:, 0]].T)       # [64][bins]
+    S1 = np.ascontiguousarray(s[s2cr[:, 1]].T)
+    if span == 1:
+        dst, nsn = np.arange(len(rno)), len(rno)
+    elif span == 11:
+        dst, nsn = np.asarray(ax['sn1_sn11'], np.int64), Cnt['NSN11']
+    else:
+        raise ValueError(f'randoms: span {span} (1 or 11)')
+    rsino = np.zeros((nsn, len(s2cr)))
+    for si in range(len(rno)):
+        rsino[dst[si]] += S0[rno[si, 0]] * S1[rno[si, 1]]
+    return np.ascontiguousarray(rsino.T), cmap, s
