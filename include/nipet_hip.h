@@ -243,6 +243,48 @@ int nipet_rnd_estimate(NipetRnd *h, const uint32_t *fansums, int niter, float *r
                        float *cmap, void *stream);
 void nipet_rnd_destroy(NipetRnd *h);
 
+/* ---- normalisation sinogram from components (mmr_auxe.norm: src/aux_module.cu:81-220,
+ * norm_from_components / dev_norm src/norm.cu:13-220; Python get_norm_sino
+ * mmrnorm.py:155-195) ----
+ * Factor of span-1 LOR (bin w + 344 a, crystals c1, c2 with gaps, rings r0, r1):
+ * geo[w] cinf[w][a % 9] E(c1, r0) E(c2, r1), E(c, r) = ceff[r][c] exp(q dtp[r] /
+ * (1 + q dtnp[r])) / (1 + q dtnp[r]), q = buckets[c / 18 + 28 (r / 8)] / 2;
+ * span 1: times axf1[si] / axe1[sn1_sn11[si]]; span 11: mean over its span-1
+ * LORs divided by axe1[s11]. */
+typedef struct NipetNorm NipetNorm;
+
+typedef struct {
+  int naw;                    /* active bins (68516) */
+  int span;                   /* 1 (4084 sinograms) or 11 (837) */
+  const int32_t *aw2ali;      /* host [naw] (txLUT['aw2ali'] = w + 344 a) */
+  const int16_t *sn1_rno;     /* host [4084][2] (axLUT['sn1_rno']) */
+  const int16_t *sn1_sn11;    /* host [4084] (axLUT['sn1_sn11']) */
+  const int8_t *sn1_sn11no;   /* host [4084] (axLUT['sn1_sn11no']) */
+} NipetNormDesc;
+
+/* Host normalisation components (mmrnorm.get_components, mmrnorm.py:19-112). */
+typedef struct {
+  const float *geo;   /* [344] geometric effects (first row of the (127, 344) array) */
+  const float *cinf;  /* [344][9] crystal interference */
+  const float *ceff;  /* [64][504] crystal efficiencies */
+  const float *axe1;  /* [837] axial effects */
+  const float *axf1;  /* [4084] span-1 axial factors (span 1 only; may be NULL for span 11) */
+  const float *dtp;   /* [64] paralysing ring dead time */
+  const float *dtnp;  /* [64] non-paralysing ring dead time */
+} NipetNormComp;
+
+int nipet_norm_create(const NipetNormDesc *desc, NipetNorm **out);
+/* out: device [naw][nsn] (the device layout). buckets: host [224] singles
+ * rates (mmrhist()['buckets']). The host arrays are consumed before return. */
+int nipet_norm_sino(NipetNorm *h, const NipetNormComp *comp, const int32_t *buckets, float *out,
+                    void *stream);
+void nipet_norm_destroy(NipetNorm *h);
+
+/* Online mean / variance update with realisation b (0-based) over n device
+ * floats (mmr_auxe.varon: src/aux_module.cu:491-556, var src/auxmath.cu:15-22):
+ * m1 += (x - m1) / (b + 1); m2 += (x - m1_old) (x - m1_new). */
+int nipet_varon(float *m1, float *m2, const float *x, int b, int64_t n, void *stream);
+
 /* Philox4x32-10 on the device for one counter/key (host arrays), for
  * known-answer checks of the bootstrap generator. Synchronous. */
 int nipet_philox4x32(const uint32_t *ctr, const uint32_t *key, uint32_t *out);

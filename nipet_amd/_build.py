@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libnipet_hip.so')
 SOURCES = ['nipet_geom.hip', 'nipet_prj.hip', 'nipet_hist.hip', 'nipet_psf.hip',
-           'nipet_rand.hip']
+           'nipet_rand.hip', 'nipet_norm.hip']
 
 HIPCC_FLAGS = [
     '--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',

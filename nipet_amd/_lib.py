@@ -41,6 +41,15 @@ class NipetRndDesc(ct.Structure):
                 ('sn1_rno', ct.c_void_p), ('sn1_sn11', ct.c_void_p)]
 
 
+class NipetNormDesc(ct.Structure):
+    _fields_ = [('naw', ct.c_int), ('span', ct.c_int), ('aw2ali', ct.c_void_p),
+                ('sn1_rno', ct.c_void_p), ('sn1_sn11', ct.c_void_p), ('sn1_sn11no', ct.c_void_p)]
+
+
+class NipetNormComp(ct.Structure):
+    _fields_ = [(k, ct.c_void_p) for k in ('geo', 'cinf', 'ceff', 'axe1', 'axf1', 'dtp', 'dtnp')]
+
+
 class NipetGeomDesc(ct.Structure):
     _fields_ = [
         ('crs', ct.c_void_p), ('ncrs', ct.c_int), ('s2c', ct.c_void_p), ('naw', ct.c_int),
@@ -87,6 +96,10 @@ def lib():
             'nipet_rnd_create': ([ct.POINTER(NipetRndDesc), ct.POINTER(P)], I),
             'nipet_rnd_estimate': ([P, P, I, P, P, P], I),
             'nipet_rnd_destroy': ([P], None),
+            'nipet_norm_create': ([ct.POINTER(NipetNormDesc), ct.POINTER(P)], I),
+            'nipet_norm_sino': ([P, ct.POINTER(NipetNormComp), P, P, P], I),
+            'nipet_norm_destroy': ([P], None),
+            'nipet_varon': ([P, P, P, I, I64, P], I),
             'nipet_philox4x32': ([P, P, P], I),
             'nipet_prof_enable': ([I], I),
             'nipet_prof_reset': ([], I),

@@ -15,6 +15,8 @@ against NIPET's ``scanner_params`` dictionary works unchanged:
   (kernels ``src/scanner_0.cu:94-166``).
 * ``get_mmrparams``  -- ``niftypet/nipet/mmraux.py:1163-1208`` (without the
   hardware mu-map discovery, which needs scanner data).
+* ``varon``          -- ``mmr_auxe.varon`` (``niftypet/nipet/src/aux_module.cu:491-556``):
+  online mean / variance update over realisations (kernel ``nipet_varon``).
 """
 import logging
 from math import pi
@@ -25,7 +27,7 @@ log = logging.getLogger(__name__)
 
 __all__ = [
     'get_mmr_constants', 'transaxial_lut', 'axial_lut', 'span1_span11', 'mmrinit',
-    'get_mmrparams', 'putgaps', 'remgaps', 'sino2ssr']
+    'get_mmrparams', 'putgaps', 'remgaps', 'sino2ssr', 'varon']
 
 
 def get_mmr_constants():
@@ -396,3 +398,36 @@ def sino2ssr(sino, axLUT, Cnt):
 
 def _is_cuda(x):
     return getattr(x, 'is_cuda', False)
+
+
+def varon(M1, M2, X, b, Cnt=None):
+    """Update the running mean ``M1`` and sum of squared deviations ``M2`` in
+    place with realisation ``X`` (0-based index ``b``), Welford's method as
+    ``src/auxmath.cu:15-22``; the variance of b+1 realisations is
+    M2 / (b + 1). float32 NumPy arrays or CUDA tensors (then asynchronous)."""
+    from . import _lib
+    import torch
+    if b < 0:
+        raise ValueError('varon: realisation index b must be >= 0')
+    if all(isinstance(a, torch.Tensor) and a.is_cuda for a in (M1, M2, X)):
+        for a in (M1, M2):
+            if a.dtype != torch.float32 or not a.is_contiguous():
+                raise ValueError('varon: M1 and M2 must be contiguous float32 tensors')
+        x = X.to(torch.float32).contiguous()
+        if not (M1.numel() == M2.numel() == x.numel()):
+            raise ValueError('varon: sizes differ')
+        _lib.check(_lib.lib().nipet_varon(M1.data_ptr(), M2.data_ptr(), x.data_ptr(), int(b),
+                                          x.numel(), _lib.stream_handle()), 'nipet_varon')
+        return M1, M2
+    for a in (M1, M2):
+        if not (isinstance(a, np.ndarray) and a.dtype == np.float32 and a.flags.c_contiguous):
+            raise ValueError('varon: M1 and M2 must be contiguous float32 arrays')
+    x = np.ascontiguousarray(np.asarray(X, np.float32))
+    if not (M1.size == M2.size == x.size):
+        raise ValueError('varon: sizes differ')
+    d = [torch.from_numpy(a.reshape(-1)).cuda() for a in (M1, M2, x)]
+    _lib.check(_lib.lib().nipet_varon(d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(), int(b),
+                                      x.size, _lib.stream_handle()), 'nipet_varon')
+    M1.reshape(-1)[:] = d[0].cpu().numpy()
+    M2.reshape(-1)[:] = d[1].cpu().numpy()
+    return M1, M2

@@ -107,8 +107,10 @@ def osemone(datain, mumaps, hst, scanner_params, recmod=3, itr=4, fwhm=0., psf=N
     ``hst['psino']`` holds the prompts (Siemens (837,252,344) or device
     (68516,837)). Corrections are arrays in either shape: ``attnsino``
     (attenuation factors; computed from ``mumaps`` by the attenuation forward
-    projector when None and recmod>0), ``normsino`` (normalisation factors,
-    ones when None), ``randsino`` (additive; estimated from ``hst['fansums']``
+    projector when None and recmod>0), ``normsino`` (normalisation factors;
+    when None, computed from ``normcomp`` or the component files named in
+    ``datain`` with ``hst['buckets']``, as ``prj/mmrrec.py:209-215``, else
+    ones), ``randsino`` (additive; estimated from ``hst['fansums']``
     when None, as ``prj/mmrrec.py:244-250``, zero without fan-sums) and
     ``sctsino`` (additive; zero when None).
     Returns a namedtuple ``(im, fpet, imsmo, fsmo, affine)`` with the E7 image.
@@ -139,7 +141,16 @@ def osemone(datain, mumaps, hst, scanner_params, recmod=3, itr=4, fwhm=0., psf=N
         mus = mmrimg.convert2dev(np.asarray(muo) + np.asarray(muh), Cnt)
         asng = mmrprj.frwd_prj(mus, scanner_params, dev_out=True, attenuation=True)
         asng = np.asarray(asng.cpu() if hasattr(asng, 'cpu') else asng)
-    nsng = np.ones(psng.shape, np.float32) if normsino is None else dev(normsino)
+    if normsino is not None:
+        nsng = dev(normsino)
+    elif normcomp is not None or (datain.get('nrm_bf') and
+                                  (datain.get('nrm_ima') or datain.get('nrm_dcm'))):
+        # prj/mmrrec.py:209-215: the normalisation sinogram from its components
+        from ..mmrnorm import NormSino, get_components
+        ncmp = normcomp if normcomp is not None else get_components(datain, Cnt)[0]
+        nsng = NormSino(scanner_params)(ncmp, hst['buckets'])
+    else:
+        nsng = np.ones(psng.shape, np.float32)
     if randsino is not None:
         rsng = dev(randsino)
     elif hst.get('fansums') is not None:

@@ -336,3 +336,56 @@ def randoms(fansums, params, niter=10, span=None, guard=True, bins=None, sino=Tr
     for si in range(len(rno)):
         rsino[dst[si]] += S0[rno[si, 0]] * S1[rno[si, 1]]
     return np.ascontiguousarray(rsino.T), cmap, s
+
+
+# ---- normalisation from components, online variance ----
+
+def norm_sino(normcomp, buckets, params, span=None, bins=None):
+    """FP64 normalisation sinogram following dev_norm (src/norm.cu:13-51):
+    per active bin aw (w = aw2ali % 344, a = aw2ali // 344, crystals with gaps
+    c1 = floor((a + (504 - 2 + 172 - w)/2) mod 504), c2 = floor((a + (1008 - 2
+    - 172 + w)/2) mod 504)) and span-1 sinogram si (rings r0, r1):
+    f = geo[0][w] cinf[w][a % 9] ceff[r0][c1] ceff[r1][c2] D(c1, r0) D(c2, r1),
+    D(c, r) = exp(q dtp[r] / (1 + q dtnp[r])) / (1 + q dtnp[r]),
+    q = buckets[c // 18 + 28 (r // 8)] / 2; span 1: f axf1[si] / axe1[s11];
+    span 11: sum over members of f / (axe1[s11] nos[si]).
+    Returns [bins][nsn] (device layout)."""
+    Cnt, tx, ax = params['Cnt'], params['txLUT'], params['axLUT']
+    span = Cnt['SPN'] if span is None else span
+    ali = np.asarray(tx['aw2ali'], np.int64)
+    if bins is not None:
+        ali = ali[np.asarray(bins, np.int64)]
+    w, a = ali % 344, ali // 344
+    c1 = np.floor(np.fmod(a + 0.5 * (504 - 2 + 172 - w), 504)).astype(np.int64)
+    c2 = np.floor(np.fmod(a + 0.5 * (2 * 504 - 2 - 172 + w), 504)).astype(np.int64)
+    g = {k: np.asarray(v, np.float64) for k, v in normcomp.items()
+         if k in ('geo', 'cinf', 'ceff', 'axe1', 'axf1', 'dtp', 'dtnp')}
+    q = 0.5 * np.asarray(buckets, np.float64).ravel()
+    r = np.arange(64)[:, None]
+    c = np.arange(504)[None, :]
+    qb = q[c // 18 + 28 * (r // 8)]                              # [64][504]
+    den = 1.0 + qb * g['dtnp'][:, None]
+    E = g['ceff'] * np.exp(qb * g['dtp'][:, None] / den) / den  # [ring][crystal]
+    T = g['geo'][0][w] * g['cinf'][w, a % 9]
+    rno = np.asarray(ax['sn1_rno'], np.int64)
+    s11 = np.asarray(ax['sn1_sn11'], np.int64)
+    nos = np.asarray(ax['sn1_sn11no'], np.float64)
+    if span == 1:
+        wsn, dst, nsn = g['axf1'] / g['axe1'][s11], np.arange(len(rno)), len(rno)
+    elif span == 11:
+        wsn, dst, nsn = 1.0 / (g['axe1'][s11] * nos), s11, Cnt['NSN11']
+    else:
+        raise ValueError(f'norm_sino: span {span}')
+    out = np.zeros((nsn, len(ali)))
+    for si in range(len(rno)):
+        out[dst[si]] += wsn[si] * E[rno[si, 0], c1] * E[rno[si, 1], c2]
+    return np.ascontiguousarray((out * T[None, :]).T)
+
+
+def varon(m1, m2, x, b):
+    """FP64 Welford update (src/auxmath.cu:15-22); returns new (m1, m2)."""
+    m1 = np.asarray(m1, np.float64)
+    x = np.asarray(x, np.float64)
+    delta = x - m1
+    n1 = m1 + delta / (b + 1)
+    return n1, np.asarray(m2, np.float64) + delta * (x - n1)
