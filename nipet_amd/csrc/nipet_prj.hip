@@ -5,16 +5,17 @@
 //    walked in chunks; for each chunk the z-columns img[col][0:127] of its
 //    voxels are staged in LDS (127 contiguous floats each: coalesced 508-byte
 //    reads) -- the projections then never gather from HBM;
-//  - all LORs of one ring difference d share their z-boundary positions along
-//    the bin (ring centres at z-voxel 2r + 0.5), so the pieces of a segment
-//    are computed once per (segment, d) and shared by the up to 64 LORs of d
-//    (reference prj/src/prjf.cu:137-190 tracks them per LOR);
-//  - FP (k_fprj): lanes own z-plane pairs, LOR sums move between lanes with
-//    DPP shifts; span-1 -> span-11 sums gathered in a fixed order, one
-//    coalesced store per sinogram bin (prj/src/prjf.cu:88-94 uses atomicAdd);
-//  - BP (k_bprj): a gather over ring differences into registers, one
-//    coalesced global float atomic per (segment, plane) instead of one per
-//    (LOR, piece) (prj/src/prjb.cu:70-82, 153-185).
+//  - FP (k_fprj): a lane owns Michelogram elements (a LOR and its mirror,
+//    which share their z boundaries) and walks the staged segments, splitting
+//    each at the next z boundary; span-1 -> span-11 sums gathered in a fixed
+//    order, one coalesced store per sinogram bin (prj/src/prjf.cu:88-94 uses
+//    atomicAdd);
+//  - BP (k_bprj): all LORs of one ring difference d share their z-boundary
+//    positions along the bin (ring centres at z-voxel 2r + 0.5), so the pieces
+//    of a segment are computed once per (segment, d); lanes own z-plane pairs
+//    and gather over d into registers, one coalesced global float atomic per
+//    (segment, plane) instead of one per (LOR, piece) (prj/src/prjb.cu:70-82,
+//    153-185).
 #include <atomic>
 #include <utility>
 
