@@ -23,6 +23,10 @@ launch, autotuned sub-batch widths); it is compared with plain
 reconstructions of single replicates (one histogram session with that
 replicate, osem_prepare, osem_run), whose images it must equal.
 
+Corrections (extra field "corrections", skipped by --no-corr): randoms from
+fan-sums, the normalisation sinogram from components and one PSF convolution
+of the phantom, each timed with HIP events.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch]
 """
 import argparse
@@ -53,6 +57,8 @@ def parse():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-batch', action='store_true', help='skip the batched bootstrap leg')
+    ap.add_argument('--no-corr', action='store_true',
+                    help='skip the randoms / normalisation / PSF timing leg')
     ap.add_argument('--nrep', type=int, default=64, help='bootstrap replicates of the batch')
     ap.add_argument('--lm-words', type=float, default=6e8, help='list-mode words (600 s)')
     ap.add_argument('--cpu-seconds', type=float, default=12.0,
@@ -171,6 +177,61 @@ def batched_bootstrap(params, d, nrep, nwords, nplain=2):
                       'sensitivity, autotune, graph capture'}
 
 
+def _event_ms(fn, reps):
+    """Mean milliseconds of fn() over reps calls, HIP events on the current
+    stream (the stream the library launches on)."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def corrections(params, d):
+    """Randoms from fan-sums, normalisation from components and the PSF
+    convolution at full size (the corrections that feed one reconstruction)."""
+    import numpy as np
+    import torch
+
+    from nipet_amd.lm.mmrhist import Randoms
+    from nipet_amd.mmrnorm import NormSino
+    from nipet_amd.prj.mmrrec import psf_config
+    from nipet_amd.prj.osem import psf_conv
+    rng = np.random.default_rng(7)
+    fan = torch.from_numpy(rng.integers(10**6, 2 * 10**6, (64, 504)).astype(np.int32)).cuda()
+    R = Randoms(params)
+    rs = torch.empty((params['txLUT']['Naw'], params['Cnt']['NSN11']), device='cuda')
+    cm = torch.empty((504, 64), device='cuda')
+    t_rnd = _event_ms(lambda: R(fan, rsino=rs, cmap=cm), 10)
+    comp = {'geo': rng.uniform(0.8, 1.2, (127, 344)), 'cinf': rng.uniform(0.9, 1.1, (344, 9)),
+            'ceff': rng.uniform(0.7, 1.3, (64, 504)), 'axe1': rng.uniform(0.9, 1.1, 837),
+            'axf1': rng.uniform(0.75, 1.17, 4084), 'dtp': rng.uniform(1e-7, 1e-6, 64),
+            'dtnp': rng.uniform(1e-7, 1e-6, 64)}
+    bk = rng.integers(10_000, 100_000, 224).astype(np.int32)
+    N = NormSino(params)
+    ns = torch.empty_like(rs)
+    t_nrm = _event_ms(lambda: N(comp, bk, out=ns), 10)
+    img = torch.from_numpy(d['img']).cuda()
+    k = psf_config(4.0, params['Cnt'])
+    po, pt = torch.empty_like(img), torch.empty_like(img)
+    t_psf = _event_ms(lambda: psf_conv(img, k, out=po, tmp=pt), 20)
+    nvox = img.numel()
+    out_bytes = rs.numel() * 4
+    return {'randoms_ms': round(t_rnd, 4), 'norm_ms': round(t_nrm, 4),
+            'psf_conv_ms': round(t_psf, 4),
+            'randoms_sino_GBs': round(out_bytes / (t_rnd * 1e-3) / 1e9, 1),
+            'norm_sino_GBs': round(out_bytes / (t_nrm * 1e-3) / 1e9, 1),
+            'psf_GBs': round(3 * 8 * nvox / (t_psf * 1e-3) / 1e9, 1),
+            'note': 'randoms: 10 fixed-point steps + span-11 sinogram (68516 x 837); norm: '
+                    'host components copied in, span-11 sinogram; psf: 3 passes over one '
+                    '320x320x127 image, 8 B/voxel/pass'}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -255,6 +316,9 @@ def main():
         'roofline': roofline,
     }
     log('byte model done')
+    if not args.no_corr:
+        log('corrections')
+        out['corrections'] = corrections(params, d)
     if not args.no_batch:
         out['batched'] = batched_bootstrap(params, d, args.nrep, args.lm_words)
     if not args.no_cpu:

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "nipet_common.h"
+#include "nipet_pairs.h"
 
 namespace nipet {
 
@@ -27,18 +28,16 @@ constexpr int NRM_NBCK = 224;
 
 struct NrmDev {
   int naw, nsn, span;
-  const short2 *cpair;     // [naw] crystals with gaps (c1, c2) of dev_norm
   const int32_t *aw2ali;   // [naw] w + 344 a
-  const short2 *sn1_rno;   // [4084]
+  const int16_t *mem_si;   // [4084] span-1 sinogram of each member (CSR order)
   const int16_t *sn1_sn11; // [4084]
   const int8_t *nos;       // [4084] span-1 sinograms in the span-11 sinogram of si
-  const int32_t *sn_off;   // [nsn + 1] span-1 members of each output sinogram
-  const int16_t *sn_mem;
   // per call
   const float *geo, *cinf, *ceff, *axe1, *axf1, *dtp, *dtnp;
   const int32_t *bck;
-  float *E;                // [64][504]
-  float *wsn;              // [4084] axial weight of each span-1 sinogram
+  float *E;                // [64][504] crystal factor E(c, r) at r * 504 + c
+  float *wmem;             // [4084] axial weight of each member (CSR order)
+  float *tw;               // [naw] geo[w] cinf[w][a % 9]
 };
 
 __global__ void k_nrm_eff(NrmDev d) {
@@ -51,26 +50,18 @@ __global__ void k_nrm_eff(NrmDev d) {
 }
 
 __global__ void k_nrm_axial(NrmDev d) {
-  const int si = blockIdx.x * blockDim.x + threadIdx.x;
-  if (si >= NRM_NSN1) return;
-  const int s11 = d.sn1_sn11[si];
-  d.wsn[si] = d.span == 1 ? d.axf1[si] / d.axe1[s11] : 1.f / (d.axe1[s11] * (float)d.nos[si]);
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= NRM_NSN1) return;
+  const int si = d.mem_si[m], s11 = d.sn1_sn11[si];
+  d.wmem[m] = d.span == 1 ? d.axf1[si] / d.axe1[s11] : 1.f / (d.axe1[s11] * (float)d.nos[si]);
 }
 
-__global__ void k_nrm_sino(float *out, NrmDev d) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)d.naw * d.nsn) return;
-  const int aw = (int)(i / d.nsn), sn = (int)(i - (int64_t)aw * d.nsn);
+__global__ void k_nrm_tx(NrmDev d) {
+  const int aw = blockIdx.x * blockDim.x + threadIdx.x;
+  if (aw >= d.naw) return;
   const int ali = d.aw2ali[aw];
   const int w = ali % NRM_W, a = ali / NRM_W;
-  const short2 cc = d.cpair[aw];
-  float acc = 0.f;
-  for (int m = d.sn_off[sn]; m < d.sn_off[sn + 1]; m++) {
-    const int si = d.sn_mem[m];
-    const short2 rr = d.sn1_rno[si];
-    acc += d.wsn[si] * (d.E[rr.x * NRM_NCRS + cc.x] * d.E[rr.y * NRM_NCRS + cc.y]);
-  }
-  out[i] = d.geo[w] * d.cinf[w * 9 + a % 9] * acc;
+  d.tw[aw] = d.geo[w] * d.cinf[w * 9 + a % 9];
 }
 
 // Welford update with instance b (0-based): M1 += (x - M1) / (b + 1),
@@ -90,6 +81,7 @@ using namespace nipet;
 
 struct NipetNorm {
   NrmDev d;
+  PairSino ps;
   std::vector<void *> alloc;
   float *comp = nullptr;  // geo | cinf | ceff | axe1 | axf1 | dtp | dtnp
   int32_t *bck = nullptr;
@@ -140,6 +132,7 @@ int nipet_norm_create(const NipetNormDesc *desc, NipetNorm **out) {
   }
   std::vector<int32_t> off(nsn + 1, 0);
   std::vector<int16_t> mem(NRM_NSN1);
+  std::vector<short2> mrr(NRM_NSN1);
   for (int si = 0; si < NRM_NSN1; si++) {
     const int r0 = desc->sn1_rno[2 * si], r1 = desc->sn1_rno[2 * si + 1];
     const int s11 = desc->sn1_sn11[si];
@@ -153,8 +146,11 @@ int nipet_norm_create(const NipetNormDesc *desc, NipetNorm **out) {
   for (int s = 0; s < nsn; s++) off[s + 1] += off[s];
   {
     std::vector<int32_t> fill(off.begin(), off.end() - 1);
-    for (int si = 0; si < NRM_NSN1; si++)
-      mem[fill[desc->span == 1 ? si : desc->sn1_sn11[si]]++] = (int16_t)si;
+    for (int si = 0; si < NRM_NSN1; si++) {
+      const int m = fill[desc->span == 1 ? si : desc->sn1_sn11[si]]++;
+      mem[m] = (int16_t)si;
+      mrr[m] = make_short2(desc->sn1_rno[2 * si], desc->sn1_rno[2 * si + 1]);
+    }
   }
   NipetNorm *h = new NipetNorm();
   NrmDev &d = h->d;
@@ -165,32 +161,31 @@ int nipet_norm_create(const NipetNormDesc *desc, NipetNorm **out) {
   int32_t *ali = nullptr, *sno = nullptr;
   int16_t *s11 = nullptr, *snm = nullptr;
   int8_t *nos = nullptr;
-  float *E = nullptr, *wsn = nullptr;
+  float *E = nullptr, *wmem = nullptr, *tw = nullptr;
   int rc;
   if ((rc = nrm_dev(h, &cpair, cp.data(), cp.size())) ||
       (rc = nrm_dev(h, &ali, desc->aw2ali, (size_t)desc->naw)) ||
-      (rc = nrm_dev(h, &rno, (const short2 *)desc->sn1_rno, (size_t)NRM_NSN1)) ||
+      (rc = nrm_dev(h, &rno, mrr.data(), mrr.size())) ||
       (rc = nrm_dev(h, &s11, desc->sn1_sn11, (size_t)NRM_NSN1)) ||
       (rc = nrm_dev(h, &nos, desc->sn1_sn11no, (size_t)NRM_NSN1)) ||
       (rc = nrm_dev(h, &sno, off.data(), off.size())) ||
       (rc = nrm_dev(h, &snm, mem.data(), mem.size())) ||
       (rc = nrm_dev(h, &E, (const float *)nullptr, (size_t)NRM_NRNG * NRM_NCRS)) ||
-      (rc = nrm_dev(h, &wsn, (const float *)nullptr, (size_t)NRM_NSN1)) ||
+      (rc = nrm_dev(h, &wmem, (const float *)nullptr, (size_t)NRM_NSN1)) ||
+      (rc = nrm_dev(h, &tw, (const float *)nullptr, (size_t)desc->naw)) ||
       (rc = nrm_dev(h, &h->comp, (const float *)nullptr, NRM_O_END)) ||
       (rc = nrm_dev(h, &h->bck, (const int32_t *)nullptr, (size_t)NRM_NBCK))) {
     nrm_free(h);
     delete h;
     return rc;
   }
-  d.cpair = cpair;
   d.aw2ali = ali;
-  d.sn1_rno = rno;
+  d.mem_si = snm;
   d.sn1_sn11 = s11;
   d.nos = nos;
-  d.sn_off = sno;
-  d.sn_mem = snm;
   d.E = E;
-  d.wsn = wsn;
+  d.wmem = wmem;
+  d.tw = tw;
   d.geo = h->comp + NRM_O_GEO;
   d.cinf = h->comp + NRM_O_CINF;
   d.ceff = h->comp + NRM_O_CEFF;
@@ -199,6 +194,8 @@ int nipet_norm_create(const NipetNormDesc *desc, NipetNorm **out) {
   d.dtp = h->comp + NRM_O_DTP;
   d.dtnp = h->comp + NRM_O_DTNP;
   d.bck = h->bck;
+  // out[aw][sn] = tw[aw] sum_m wmem[m] E[r0][c1] E[r1][c2]
+  h->ps = PairSino{E, 1, NRM_NCRS, cpair, tw, sno, rno, wmem, desc->naw, nsn};
   *out = h;
   return NIPET_OK;
 }
@@ -232,8 +229,8 @@ int nipet_norm_sino(NipetNorm *h, const NipetNormComp *c, const int32_t *buckets
   NIPET_HIP(hipStreamSynchronize(st));
   k_nrm_eff<<<(NRM_NRNG * NRM_NCRS + 255) / 256, 256, 0, st>>>(h->d);
   k_nrm_axial<<<(NRM_NSN1 + 255) / 256, 256, 0, st>>>(h->d);
-  const int64_t m = (int64_t)h->d.naw * h->d.nsn;
-  k_nrm_sino<<<(unsigned)((m + 255) / 256), 256, 0, st>>>(out, h->d);
+  k_nrm_tx<<<(h->d.naw + 255) / 256, 256, 0, st>>>(h->d);
+  k_pair_sino<<<pair_blocks(h->d.naw), PAIR_THREADS, 0, st>>>(out, h->ps);
   NIPET_HIP(hipGetLastError());
   return NIPET_OK;
 }

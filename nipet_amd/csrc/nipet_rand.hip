@@ -4,29 +4,32 @@
 // Crystal singles rates s[c][r] (448 active crystals x 64 rings) are found from
 // the delayed fan-sums f by the reference's fixed-point iteration
 //   s <- s / 2 + f / (2 F s),   s0 = sqrt(f / F 1),   10 iterations,
-// where (F s)[c][r] sums s over the crystals c' in coincidence with c (cij)
-// and the rings r' with |r' - r| <= MRD. F runs as a ring-band prefix sum
-// followed by a 448 x 448 crystal product (cij row in LDS). The randoms
-// sinogram is s[c0][r0] s[c1][r1] per span-1 LOR, and span-11 bins sum their
-// span-1 LORs in a fixed order (the reference uses float atomics).
+// where (F s)[c][r] sums s over the crystals in coincidence with c and the
+// rings r' with |r' - r| <= MRD. As in the reference (lm/src/rnd.cu:120-150,
+// 205-226), the crystals in coincidence with c are the circular run of ones of
+// row c of cij, walked from its first crystal for at most 320 crystals.
+// One kernel per iteration: a block per crystal c sums the run for all 64
+// rings (4 interleaved partial sums per ring, combined in a fixed order), then
+// the ring band, then updates s (double-buffered: other blocks still read the
+// previous iterate). The randoms sinogram of LOR (c0, r0; c1, r1) is
+// s[c0][r0] s[c1][r1]; span-11 bins sum their span-1 LORs in a fixed order
+// (the reference uses float atomics).
 #include <vector>
 
 #include "nipet_common.h"
+#include "nipet_pairs.h"
 
 namespace nipet {
 
-constexpr int RND_NR = 64;    // rings
-constexpr int RND_NC = 448;   // active crystals
-constexpr int RND_NCRS = 504; // crystals with gaps
+constexpr int RND_NR = 64;     // rings
+constexpr int RND_NC = 448;    // active crystals
+constexpr int RND_NCRS = 504;  // crystals with gaps
+constexpr int RND_WALK = 320;  // crystals walked per fan (16 x 20 threads, lm/src/rnd.cu:126-131)
 
 struct RndDev {
-  int naw, nsn, mrd;
-  const int8_t *cij;       // [448][448]
+  int mrd;
+  const short2 *run;       // [448] first crystal and length of the coincidence run
   const int16_t *crsri;    // [504] reduced crystal or -1
-  const short2 *s2cr;      // [naw] reduced crystal pair
-  const short2 *sn1_rno;   // [4084] ring pair of each span-1 sinogram
-  const int32_t *sn_off;   // [nsn + 1] span-1 sinograms of each output sinogram
-  const int16_t *sn_mem;
 };
 
 // fansums [64][504] (gaps included) -> f[c][r] over active crystals
@@ -38,60 +41,40 @@ __global__ void k_rnd_fsum(float *f, const uint32_t *fan, const int16_t *crsri) 
   if (cr >= 0) f[cr * RND_NR + r] = (float)fan[i];
 }
 
-// t[c][r] = sum over rings r' with |r' - r| <= mrd of s[c][r'] (block per c)
-__global__ void k_rnd_rings(float *t, const float *s, int mrd) {
-  __shared__ float v[RND_NR];
-  const int c = blockIdx.x, r = threadIdx.x;
-  v[r] = s[c * RND_NR + r];
-  __syncthreads();
-  const int a = r - mrd < 0 ? 0 : r - mrd, b = r + mrd > RND_NR - 1 ? RND_NR - 1 : r + mrd;
-  float acc = 0.f;
-  for (int q = a; q <= b; q++) acc += v[q];
-  t[c * RND_NR + r] = acc;
-}
-
-// out[c][r] = sum over crystals c' with cij[c][c'] of t[c'][r] (block per c,
-// thread per ring; crystals summed in ascending order)
-__global__ void k_rnd_crys(float *out, const float *t, const int8_t *cij) {
-  __shared__ int8_t row[RND_NC];
-  const int c = blockIdx.x, r = threadIdx.x;
-  for (int i = r; i < RND_NC; i += RND_NR) row[i] = cij[c * RND_NC + i];
-  __syncthreads();
-  float acc = 0.f;
-  for (int q = 0; q < RND_NC; q++)
-    if (row[q]) acc += t[q * RND_NR + r];
-  out[c * RND_NR + r] = acc;
-}
-
-__global__ void k_rnd_ones(float *s) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < RND_NC * RND_NR) s[i] = 1.f;
-}
-
-// init: s = sqrt(f / F1); step: s = s / 2 + (Fs > 0 ? f / Fs : 0) / 2
-__global__ void k_rnd_update(float *s, const float *f, const float *fs, int init) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= RND_NC * RND_NR) return;
-  if (init) {
-    s[i] = fs[i] > 0.f ? sqrtf(f[i] / fs[i]) : 0.f;
-  } else {
-    const float q = fs[i] > 0.f ? f[i] / fs[i] : 0.f;
-    s[i] = 0.5f * s[i] + 0.5f * q;
+// INIT: so = sqrt(f / F 1) (F 1 = run length x ring-band length);
+// else: so = si / 2 + (F si > 0 ? f / F si : 0) / 2. Block = crystal c, 256
+// threads = 4 partial sums x 64 rings.
+template <bool INIT>
+__global__ __launch_bounds__(256) void k_rnd_step(float *__restrict__ so,
+                                                  const float *__restrict__ si,
+                                                  const float *__restrict__ f, RndDev d) {
+  __shared__ float part[4][RND_NR];
+  __shared__ float u[RND_NR];
+  const int c = blockIdx.x, g = threadIdx.x >> 6, r = threadIdx.x & 63;
+  const short2 run = d.run[c];
+  const int lo = r - d.mrd < 0 ? 0 : r - d.mrd;
+  const int hi = r + d.mrd > RND_NR - 1 ? RND_NR - 1 : r + d.mrd;
+  const int i = c * RND_NR + r;
+  if (INIT) {
+    if (g == 0) so[i] = sqrtf(f[i] / (float)(run.y * (hi - lo + 1)));
+    return;
   }
-}
-
-// randoms sinogram, device layout out[aw][sn]
-__global__ void k_rnd_sino(float *out, const float *s, RndDev d) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)d.naw * d.nsn) return;
-  const int aw = (int)(i / d.nsn), sn = (int)(i - (int64_t)aw * d.nsn);
-  const short2 cc = d.s2cr[aw];
   float acc = 0.f;
-  for (int m = d.sn_off[sn]; m < d.sn_off[sn + 1]; m++) {
-    const short2 rr = d.sn1_rno[d.sn_mem[m]];
-    acc += s[cc.x * RND_NR + rr.x] * s[cc.y * RND_NR + rr.y];
+  for (int k = g; k < run.y; k += 4) {
+    int q = run.x + k;
+    if (q >= RND_NC) q -= RND_NC;
+    acc += si[q * RND_NR + r];
   }
-  out[i] = acc;
+  part[g][r] = acc;
+  __syncthreads();
+  if (g == 0) u[r] = (part[0][r] + part[1][r]) + (part[2][r] + part[3][r]);
+  __syncthreads();
+  if (g == 0) {
+    float fs = 0.f;
+    for (int q = lo; q <= hi; q++) fs += u[q];
+    const float qv = fs > 0.f ? f[i] / fs : 0.f;
+    so[i] = 0.5f * si[i] + 0.5f * qv;
+  }
 }
 
 // crystal map with gaps, cmap[c][r] (504 x 64)
@@ -109,23 +92,21 @@ using namespace nipet;
 
 struct NipetRnd {
   RndDev d;
-  float *f = nullptr, *s = nullptr, *t = nullptr, *fs = nullptr;
-  void *alloc[8] = {};
+  PairSino ps;
+  float *f = nullptr, *s[2] = {nullptr, nullptr};
+  std::vector<void *> alloc;
 };
 
 static void rnd_free(NipetRnd *h) {
   for (void *p : h->alloc) (void)hipFree(p);
-  (void)hipFree(h->f);
-  (void)hipFree(h->s);
-  (void)hipFree(h->t);
-  (void)hipFree(h->fs);
+  h->alloc.clear();
 }
 
 template <typename T>
-static int rnd_upload(NipetRnd *h, int slot, T **dst, const T *src, size_t n) {
+static int rnd_dev(NipetRnd *h, T **dst, const T *src, size_t n) {
   NIPET_HIP(hipMalloc((void **)dst, n * sizeof(T)));
-  h->alloc[slot] = *dst;
-  NIPET_HIP(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+  h->alloc.push_back(*dst);
+  if (src) NIPET_HIP(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
   return NIPET_OK;
 }
 
@@ -148,6 +129,27 @@ int nipet_rnd_create(const NipetRndDesc *desc, NipetRnd **out) {
       return NIPET_EINVAL;
     }
   }
+  // coincidence run of each crystal: first crystal after a 0 -> 1 step of row
+  // c of cij (circularly), its length, walked for at most 320 crystals
+  std::vector<short2> run(RND_NC);
+  for (int c = 0; c < RND_NC; c++) {
+    const int8_t *row = desc->cij + (size_t)c * RND_NC;
+    int nup = 0, start = 0, len = 0;
+    for (int q = 0; q < RND_NC; q++) {
+      const int prv = row[(q + RND_NC - 1) % RND_NC] != 0, cur = row[q] != 0;
+      if (cur && !prv) {
+        nup++;
+        start = q;
+      }
+      len += cur;
+    }
+    if (nup != 1) {
+      set_error("nipet_rnd_create: row %d of cij is not one circular run of crystals", c);
+      return NIPET_EINVAL;
+    }
+    run[c].x = (short)start;
+    run[c].y = (short)(len < RND_WALK ? len : RND_WALK);
+  }
   for (int i = 0; i < 2 * desc->naw; i++)
     if (desc->s2cr[i] < 0 || desc->s2cr[i] >= RND_NC) {
       set_error("nipet_rnd_create: s2cr entry %d out of range", i);
@@ -158,9 +160,9 @@ int nipet_rnd_create(const NipetRndDesc *desc, NipetRnd **out) {
       set_error("nipet_rnd_create: sn1_rno entry %d out of range", i);
       return NIPET_EINVAL;
     }
-  // span-1 members of each output sinogram
+  // span-1 members of each output sinogram, ascending
   std::vector<int32_t> off(desc->nsn + 1, 0);
-  std::vector<int16_t> mem(4084);
+  std::vector<short2> mem(4084);
   for (int si = 0; si < 4084; si++) {
     const int s = desc->nsn == 4084 ? si : desc->sn1_sn11[si];
     if (s < 0 || s >= desc->nsn) {
@@ -174,44 +176,31 @@ int nipet_rnd_create(const NipetRndDesc *desc, NipetRnd **out) {
     std::vector<int32_t> fill(off.begin(), off.end() - 1);
     for (int si = 0; si < 4084; si++) {
       const int s = desc->nsn == 4084 ? si : desc->sn1_sn11[si];
-      mem[fill[s]++] = (int16_t)si;
+      mem[fill[s]++] = make_short2(desc->sn1_rno[2 * si], desc->sn1_rno[2 * si + 1]);
     }
   }
   NipetRnd *h = new NipetRnd();
-  h->d.naw = desc->naw;
-  h->d.nsn = desc->nsn;
-  h->d.mrd = desc->mrd;
-  int8_t *cij = nullptr;
-  int16_t *crsri = nullptr, *snm = nullptr;
-  short2 *s2cr = nullptr, *rno = nullptr;
-  int32_t *sno = nullptr;
-  int rc = NIPET_OK;
-  if ((rc = rnd_upload(h, 0, &cij, desc->cij, (size_t)RND_NC * RND_NC)) ||
-      (rc = rnd_upload(h, 1, &crsri, desc->crsri, (size_t)RND_NCRS)) ||
-      (rc = rnd_upload(h, 2, &s2cr, (const short2 *)desc->s2cr, (size_t)desc->naw)) ||
-      (rc = rnd_upload(h, 3, &rno, (const short2 *)desc->sn1_rno, (size_t)4084)) ||
-      (rc = rnd_upload(h, 4, &sno, off.data(), off.size())) ||
-      (rc = rnd_upload(h, 5, &snm, mem.data(), mem.size()))) {
+  short2 *drun = nullptr, *s2cr = nullptr, *mrr = nullptr;
+  int16_t *crsri = nullptr;
+  int32_t *doff = nullptr;
+  const size_t ns = (size_t)RND_NC * RND_NR;
+  int rc;
+  if ((rc = rnd_dev(h, &drun, run.data(), run.size())) ||
+      (rc = rnd_dev(h, &crsri, desc->crsri, (size_t)RND_NCRS)) ||
+      (rc = rnd_dev(h, &s2cr, (const short2 *)desc->s2cr, (size_t)desc->naw)) ||
+      (rc = rnd_dev(h, &doff, off.data(), off.size())) ||
+      (rc = rnd_dev(h, &mrr, mem.data(), mem.size())) ||
+      (rc = rnd_dev(h, &h->f, (const float *)nullptr, ns)) ||
+      (rc = rnd_dev(h, &h->s[0], (const float *)nullptr, ns)) ||
+      (rc = rnd_dev(h, &h->s[1], (const float *)nullptr, ns))) {
     rnd_free(h);
     delete h;
     return rc;
   }
-  h->d.cij = cij;
+  h->d.mrd = desc->mrd;
+  h->d.run = drun;
   h->d.crsri = crsri;
-  h->d.s2cr = s2cr;
-  h->d.sn1_rno = rno;
-  h->d.sn_off = sno;
-  h->d.sn_mem = snm;
-  const size_t n = (size_t)RND_NC * RND_NR * sizeof(float);
-  hipError_t e = hipMalloc(&h->f, n);
-  if (e == hipSuccess) e = hipMalloc(&h->s, n);
-  if (e == hipSuccess) e = hipMalloc(&h->t, n);
-  if (e == hipSuccess) e = hipMalloc(&h->fs, n);
-  if (e != hipSuccess) {
-    rnd_free(h);
-    delete h;
-    return hip_fail(e, "nipet_rnd_create");
-  }
+  h->ps = PairSino{nullptr, RND_NR, 1, s2cr, nullptr, doff, mrr, nullptr, desc->naw, desc->nsn};
   *out = h;
   return NIPET_OK;
 }
@@ -223,25 +212,18 @@ int nipet_rnd_estimate(NipetRnd *h, const uint32_t *fansums, int niter, float *r
     return NIPET_EINVAL;
   }
   hipStream_t st = (hipStream_t)stream;
-  const int n = RND_NC * RND_NR;
-  const unsigned nbl = (n + 255) / 256;
   k_rnd_fsum<<<(RND_NR * RND_NCRS + 255) / 256, 256, 0, st>>>(h->f, fansums, h->d.crsri);
-  // F 1 -> fs, init s
-  k_rnd_ones<<<nbl, 256, 0, st>>>(h->s);
-  k_rnd_rings<<<RND_NC, RND_NR, 0, st>>>(h->t, h->s, h->d.mrd);
-  k_rnd_crys<<<RND_NC, RND_NR, 0, st>>>(h->fs, h->t, h->d.cij);
-  k_rnd_update<<<nbl, 256, 0, st>>>(h->s, h->f, h->fs, 1);
-  for (int k = 0; k < niter; k++) {
-    k_rnd_rings<<<RND_NC, RND_NR, 0, st>>>(h->t, h->s, h->d.mrd);
-    k_rnd_crys<<<RND_NC, RND_NR, 0, st>>>(h->fs, h->t, h->d.cij);
-    k_rnd_update<<<nbl, 256, 0, st>>>(h->s, h->f, h->fs, 0);
-  }
+  k_rnd_step<true><<<RND_NC, 256, 0, st>>>(h->s[0], h->s[1], h->f, h->d);
+  int cur = 0;
+  for (int k = 0; k < niter; k++, cur ^= 1)
+    k_rnd_step<false><<<RND_NC, 256, 0, st>>>(h->s[cur ^ 1], h->s[cur], h->f, h->d);
   if (rsino) {
-    const int64_t m = (int64_t)h->d.naw * h->d.nsn;
-    k_rnd_sino<<<(unsigned)((m + 255) / 256), 256, 0, st>>>(rsino, h->s, h->d);
+    PairSino p = h->ps;
+    p.tab = h->s[cur];
+    k_pair_sino<<<pair_blocks(p.naw), PAIR_THREADS, 0, st>>>(rsino, p);
   }
   if (cmap)
-    k_rnd_cmap<<<(RND_NCRS * RND_NR + 255) / 256, 256, 0, st>>>(cmap, h->s, h->d.crsri);
+    k_rnd_cmap<<<(RND_NCRS * RND_NR + 255) / 256, 256, 0, st>>>(cmap, h->s[cur], h->d.crsri);
   NIPET_HIP(hipGetLastError());
   return NIPET_OK;
 }
