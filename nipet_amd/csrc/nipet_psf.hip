@@ -23,6 +23,8 @@ struct PsfTaps {
   float k[PSF_TAPS];
 };
 
+constexpr int PSF_SEG = 80;  // line piece per thread along x and y (multiple of 8)
+
 // z-lines: line l = (column a, replicate b) at base a * NZ * nb + b, stride nb.
 constexpr int PSF_ZLINES = 2;  // lines per 256-thread block (128 threads each)
 
@@ -49,28 +51,33 @@ __global__ __launch_bounds__(256) void k_psf_z(float *__restrict__ out,
 }
 
 // Strided axis of length n: element (a, i, c) at a * n * inner + i * inner + c.
+// blockIdx.y selects the piece [y seg, (y + 1) seg) of every line (seg a
+// multiple of 8): more threads in flight for the same traffic plus 16 halo
+// reads per piece.
 __global__ __launch_bounds__(256) void k_psf_line(float *__restrict__ out,
                                                   const float *__restrict__ in, PsfTaps t,
-                                                  int n, int64_t inner, int64_t nlines) {
+                                                  int n, int64_t inner, int64_t nlines,
+                                                  int seg) {
   const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= nlines) return;
   const int64_t a = l / inner, c = l - a * inner;
   const float *src = in + a * n * inner + c;
   float *dst = out + a * n * inner + c;
+  const int ib = blockIdx.y * seg, ie = min(n, ib + seg);
   // w[q] = in[i0 - 8 + q] for the current block of 8 outputs starting at i0
   float w[24];
 #pragma unroll
   for (int q = 0; q < 24; q++) {
-    const int i = q - 8;
+    const int i = ib + q - 8;
     w[q] = (i >= 0 && i < n) ? src[(int64_t)i * inner] : 0.f;
   }
-  for (int i0 = 0; i0 < n; i0 += 8) {
+  for (int i0 = ib; i0 < ie; i0 += 8) {
 #pragma unroll
     for (int k = 0; k < 8; k++) {
       float s = 0.f;
 #pragma unroll
       for (int j = -PSF_R; j <= PSF_R; j++) s += t.k[PSF_R - j] * w[8 + k + j];
-      if (i0 + k < n) dst[(int64_t)(i0 + k) * inner] = s;
+      if (i0 + k < ie) dst[(int64_t)(i0 + k) * inner] = s;
     }
 #pragma unroll
     for (int q = 0; q < 16; q++) w[q] = w[q + 8];
@@ -94,9 +101,11 @@ int psf_conv(float *out, const float *in, float *tmp, const float *krnl, int nb,
   const int64_t zl = (int64_t)NX * NY * nb;
   k_psf_z<<<(unsigned)((zl + PSF_ZLINES - 1) / PSF_ZLINES), 256, 0, st>>>(out, in, tz, zl, nb);
   const int64_t in_x = (int64_t)NZ * nb, lx = (int64_t)NY * in_x;
-  k_psf_line<<<(unsigned)((lx + 255) / 256), 256, 0, st>>>(tmp, out, tx, NX, in_x, lx);
+  const dim3 gx((unsigned)((lx + 255) / 256), (NX + PSF_SEG - 1) / PSF_SEG);
+  k_psf_line<<<gx, 256, 0, st>>>(tmp, out, tx, NX, in_x, lx, PSF_SEG);
   const int64_t in_y = (int64_t)NX * NZ * nb;
-  k_psf_line<<<(unsigned)((in_y + 255) / 256), 256, 0, st>>>(out, tmp, ty, NY, in_y, in_y);
+  const dim3 gy((unsigned)((in_y + 255) / 256), (NY + PSF_SEG - 1) / PSF_SEG);
+  k_psf_line<<<gy, 256, 0, st>>>(out, tmp, ty, NY, in_y, in_y, PSF_SEG);
   return NIPET_OK;
 }
 
