@@ -44,6 +44,8 @@ struct GeomView {
   const int16_t *dr_li;
   const int32_t *sn_off;
   const int16_t *sn_mem;
+  const int16_t *fp_slot[2];
+  int fp_nslot[2];
   int naw, nli, ndirect, nsn, dmax;
   float vox_z;
 };
@@ -60,6 +62,10 @@ static GeomView view(const Geom *g) {
   v.dr_li = g->dr_li;
   v.sn_off = g->sn_off;
   v.sn_mem = g->sn_mem;
+  for (int i = 0; i < 2; i++) {
+    v.fp_slot[i] = g->fp_slot[i];
+    v.fp_nslot[i] = g->fp_nslot[i];
+  }
   v.naw = g->naw;
   v.nli = g->nli;
   v.ndirect = g->ndirect;
@@ -381,6 +387,267 @@ __global__ __launch_bounds__(FP_THREADS, NB >= 4 ? 4 : 6) void k_fprj(GeomView g
     }
     __syncthreads();
     for (int s = tid; s < nsn; s += FP_THREADS) {
+      float v = 0.f;
+      const int m1 = g.sn_off[s + 1];
+      for (int m = g.sn_off[s]; m < m1; m++) v += lorv[g.sn_mem[m]];
+      if (MODE == 2) {
+        const size_t kk = (size_t)aw * nsn + s;
+        const float ee = v + (rho_nb ? rho[kk * nbt + b0 + b] : rho[kk]);
+        v = (ee > -EPS && ee < EPS) ? 0.f : (float)prompts[kk * nbt + b0 + b] / ee;
+      }
+      out[(size_t)s * nbt + b] = v;
+    }
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Forward projection with lanes over groups of G LORs of one ring difference.
+//
+// LOR (d, r0 + k) lies exactly 2k planes above LOR (d, r0) along the whole ray
+// (ring centres at zeta = 2 r + 0.5), so the G LORs of a slot (and their
+// mirrors, 2k planes above the mirror of (d, r0)) share the boundary
+// positions, the pieces p1 / p2 and the plane steps: per segment a lane
+// computes those once and adds 2 (4 across a boundary) products per LOR pair,
+// reading the G planes at immediate offsets from one pointer. Per LOR the
+// arithmetic equals k_fprj's, so both kernels give the same values. Slots (d, r0) come from the
+// geometry (fp_slot); lanes whose group runs past ring 63 walk phantom LORs
+// whose planes lie in zero guard planes and are never written out.
+// G is odd: lanes of one ring difference then sit 2G planes apart, which puts
+// them on distinct LDS banks (plane stride 17 floats; an even G would fold
+// them onto 8 or 16 banks).
+template <int G>
+constexpr int fpg_threads() { return G == 3 ? 768 : 448; }  // >= 712 (G = 3) / 439 (G = 5) slots
+template <int G>
+constexpr int fpg_rounds() { return 1; }
+template <int G>
+constexpr int fpg_top() { return 2 + 2 * (G - 1); }    // zero planes above 127
+template <int G>
+constexpr int fpg_planes() { return FP_GUARD + ZSTRIDE + fpg_top<G>(); }
+
+template <int NB, int G>
+struct FpSumG {
+  float pa[G][NB], pb[G][NB], ta[G][NB], tb[G][NB];
+  __device__ void fold() {
+    static_for<G>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        ta[k][b] += pa[k][b];
+        tb[k][b] += pb[k][b];
+        pa[k][b] = 0.f;
+        pb[k][b] = 0.f;
+      }
+    });
+  }
+};
+
+template <int NB, int G, bool TWO>
+__device__ inline void fpg_walk(FpLor &l, FpSumG<NB, G> &acc, const float2 *__restrict__ sr,
+                                const float *__restrict__ zs, int nch) {
+  constexpr int PS = fp_ps<NB>() * NB;  // plane stride (floats)
+  for (int j8 = 0; j8 < nch; j8 += SUBSUM) {
+    const float *za = zs + l.ao + j8 * NB;
+    const float *zb = zs + l.bo + j8 * NB;
+#pragma unroll(NB >= 2 ? 2 : 4)
+    for (int jj = 0; jj < SUBSUM; jj++) {
+      const float2 sl = sr[j8 + jj];
+      if constexpr (!TWO) {
+        const float q = l.tbr - sl.x;
+        const float p1 = __builtin_amdgcn_fmed3f(q, 0.f, sl.y);
+        const bool cr = q <= sl.y;
+        static_for<G>([&](auto kc) {
+          constexpr int k = decltype(kc)::value;
+          float xa0[NB], xb0[NB];
+          ldv<NB>(xa0, za + 2 * k * PS + jj * NB);
+          ldv<NB>(xb0, zb + (2 * k + 2) * PS + jj * NB);
+          fma_nb<NB>(acc.pa[k], p1, xa0);
+          fma_nb<NB>(acc.pb[k], p1, xb0);
+        });
+        // second piece only where the segment holds a boundary: the same
+        // segment for every lane of a ring difference, so whole lane groups
+        // skip these plane reads together
+        if (cr) {
+          const float p2 = sl.y - p1;
+          static_for<G>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            float xa1[NB], xb1[NB];
+            ldv<NB>(xa1, za + (2 * k + 1) * PS + jj * NB);
+            ldv<NB>(xb1, zb + (2 * k + 1) * PS + jj * NB);
+            fma_nb<NB>(acc.pa[k], p2, xa1);
+            fma_nb<NB>(acc.pb[k], p2, xb1);
+          });
+        }
+        const int step = cr ? PS : 0;
+        za += step;
+        zb -= step;
+        l.tbr += cr ? l.invf : 0.f;
+      } else {
+        const float q1 = l.tbr - sl.x, q2 = q1 + l.invf;
+        const float p1 = __builtin_amdgcn_fmed3f(q1, 0.f, sl.y);
+        const float c2 = __builtin_amdgcn_fmed3f(q2, 0.f, sl.y);
+        const float p2 = c2 - p1, p3 = sl.y - c2;
+        static_for<G>([&](auto kc) {
+          constexpr int k = decltype(kc)::value;
+          float xa0[NB], xa1[NB], xa2[NB], xb0[NB], xb1[NB], xb2[NB];
+          ldv<NB>(xa0, za + 2 * k * PS + jj * NB);
+          ldv<NB>(xa1, za + (2 * k + 1) * PS + jj * NB);
+          ldv<NB>(xa2, za + (2 * k + 2) * PS + jj * NB);
+          ldv<NB>(xb0, zb + (2 * k + 2) * PS + jj * NB);
+          ldv<NB>(xb1, zb + (2 * k + 1) * PS + jj * NB);
+          ldv<NB>(xb2, zb + 2 * k * PS + jj * NB);
+          fma_nb<NB>(acc.pa[k], p1, xa0);
+          fma_nb<NB>(acc.pa[k], p2, xa1);
+          fma_nb<NB>(acc.pa[k], p3, xa2);
+          fma_nb<NB>(acc.pb[k], p1, xb0);
+          fma_nb<NB>(acc.pb[k], p2, xb1);
+          fma_nb<NB>(acc.pb[k], p3, xb2);
+        });
+        const int n = (int)(q1 <= sl.y) + (int)(q2 <= sl.y);
+        za += n * PS;
+        zb -= n * PS;
+        l.tbr += (float)n * l.invf;
+      }
+    }
+    l.ao = (int)(za - zs) - j8 * NB;
+    l.bo = (int)(zb - zs) - j8 * NB;
+    acc.fold();
+  }
+}
+
+template <int MODE, int NB, int G>
+__global__ __launch_bounds__(fpg_threads<G>()) void k_fprj_g(GeomView g, const float *__restrict__ img,
+                                                           float *__restrict__ sino,
+                                                           const int32_t *__restrict__ subs,
+                                                           const uint16_t *__restrict__ prompts,
+                                                           const float *__restrict__ rho,
+                                                           int rho_nb, Batch bt) {
+  constexpr int CH = fp_chunk<NB>(), PS = fp_ps<NB>();
+  constexpr int R = fpg_rounds<G>(), NP = fpg_planes<G>(), FPG_THREADS = fpg_threads<G>();
+  constexpr int GI = G == FP_GROUPS[0] ? 0 : 1;
+  constexpr int ZN = NP * PS * NB, LN = 2 * LI_MAX;
+  __shared__ __align__(16) float lds[ZN > LN ? ZN : LN];
+  __shared__ float2 sr[CH];
+  __shared__ double inv[R][FPG_THREADS];  // exact boundary spacing (0: direct)
+  __shared__ float z0[R][FPG_THREADS];    // zeta of the slot's first LOR at t = 0
+  const int tid = threadIdx.x, wave = tid >> 6;
+  const int row = blockIdx.x;
+  const int aw = subs ? subs[row] : row;
+  const int nsn = g.nsn, nbt = bt.nbt, b0 = bt.b0;
+  const int nslot = g.fp_nslot[GI];
+  const int16_t *slots = g.fp_slot[GI];
+  float *out = sino + (size_t)row * nsn * nbt + b0;
+  if (aw < 0 || aw >= g.naw) {  // invalid bin index: zero row
+    for (int i = tid; i < nsn * NB; i += FPG_THREADS) out[(i / NB) * nbt + i % NB] = 0.f;
+    return;
+  }
+  const int beg = g.off[aw], nseg = g.off[aw + 1] - beg;
+  const double atn = g.atn[aw];
+  const float lmax = g.lmax[aw];
+  float *zs = lds;
+  // guard planes (never written by the staging loop)
+  for (int i = tid; i < FP_GUARD * PS * NB; i += FPG_THREADS) zs[i] = 0.f;
+  for (int i = tid; i < fpg_top<G>() * PS * NB; i += FPG_THREADS)
+    zs[(FP_GUARD + ZSTRIDE) * PS * NB + i] = 0.f;
+
+  FpLor L[R];
+  FpSumG<NB, G> A[R];
+  bool two[R], live[R];
+  int dr[R];  // d * 64 + r0 of the slot
+  const double s0 = nseg > 0 ? g.seg[beg].s : 0.0;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int sl = r * FPG_THREADS + tid;
+    dr[r] = sl < nslot ? slots[sl] : 0;
+    const int d = dr[r] >> 6, r0 = dr[r] & 63;
+    const float2 z = make_float2(2.f * r0 + 0.5f, 2.f * d);
+    const int wsum = (int)floorf(z.x) + (int)floorf(z.x + z.y);
+    int ia;
+    double iv = 0.0;
+    if (z.y > 0.f) {
+      iv = atn / (double)z.y;
+      ia = (int)floor((double)z.x + s0 / iv);
+    } else {
+      ia = (int)floorf(z.x);
+    }
+    z0[r][tid] = z.x;
+    inv[r][tid] = iv;
+    L[r].ao = (ia + FP_GUARD) * PS * NB;
+    L[r].bo = (wsum - ia - 2 + FP_GUARD) * PS * NB;
+    L[r].invf = (float)iv;
+    static_for<G>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+#pragma unroll
+      for (int b = 0; b < NB; b++) A[r].pa[k][b] = A[r].pb[k][b] = A[r].ta[k][b] = A[r].tb[k][b] = 0.f;
+    });
+    live[r] = r * FPG_THREADS + wave * 64 < nslot;  // wave-uniform
+    two[r] = __any(iv > 0.0 && iv < (double)lmax);
+  }
+
+  for (int c = 0; c < nseg; c += CH) {
+    const int nch = nseg - c < CH ? nseg - c : CH;
+    const double sb = g.seg[beg + c].s;  // chunk base position
+    __syncthreads();
+    if (tid < CH) {
+      const SegRec sg = g.seg[beg + (c + tid < nseg ? c + tid : c)];
+      sr[tid] = c + tid < nseg ? make_float2((float)(sg.s - sb), sg.len) : make_float2(-1e30f, 0.f);
+    }
+    for (int idx = tid; idx < CH * ZSTRIDE * NB; idx += FPG_THREADS) {
+      const int j = idx / (ZSTRIDE * NB), rem = idx - j * (ZSTRIDE * NB);
+      const int z = rem / NB, b = rem - z * NB, k = c + j;
+      float v = 0.f;
+      if (k < nseg && z < NZ) v = img[((int64_t)g.seg[beg + k].colz + z) * nbt + b0 + b];
+      zs[((z + FP_GUARD) * PS + j) * NB + b] = v;
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int ia = L[r].ao / (PS * NB) - FP_GUARD;
+      const double iv = inv[r][tid];
+      L[r].tbr = iv > 0.0 ? (float)(((double)(ia + 1) - (double)z0[r][tid]) * iv - sb) : 3e38f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if (!live[r]) continue;
+      if (two[r])
+        fpg_walk<NB, G, true>(L[r], A[r], sr, zs, nch);
+      else
+        fpg_walk<NB, G, false>(L[r], A[r], sr, zs, nch);
+    }
+  }
+
+  // per-LOR values -> LDS; each output bin then sums its LORs in CSR order
+  float *lorv = lds;
+  float sf[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int d = dr[r] >> 6;
+    sf[r] = lor_sfac(2.f * d, g.vox_z, atn);
+    A[r].fold();
+  }
+  static_for<NB>([&](auto bc) {
+    constexpr int b = decltype(bc)::value;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if (r * FPG_THREADS + tid >= nslot) continue;
+      const int d = dr[r] >> 6, r0 = dr[r] & 63;
+      static_for<G>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const int li = r0 + k + d <= 63 ? g.dr_li[dr[r] + k] : -1;
+        if (li >= 0) {
+          float va = A[r].ta[k][b] * sf[r], vb = A[r].tb[k][b] * sf[r];
+          if (MODE == 1) {
+            const float nosi = g.li_nosi[li];
+            va = expf(-va) * nosi;
+            vb = expf(-vb) * nosi;
+          }
+          lorv[2 * li] = va;
+          if (d > 0) lorv[2 * li + 1] = vb;
+        }
+      });
+    }
+    __syncthreads();
+    for (int s = tid; s < nsn; s += FPG_THREADS) {
       float v = 0.f;
       const int m1 = g.sn_off[s + 1];
       for (int m = g.sn_off[s]; m < m1; m++) v += lorv[g.sn_mem[m]];
@@ -755,6 +1022,7 @@ using namespace nipet;
 // Largest replicate sub-batch per kernel instance (4, 2 or 1), set by the
 // autotuner through nipet_tune_set; process-wide.
 static std::atomic<int> g_width_fp{4}, g_width_bp{4};
+constexpr int FP_G1 = 3;  // LORs per lane of the single-image forward projector
 
 template <int MODE>
 static void launch_fprj(const Geom *g, float *sino, const float *img, const int32_t *subs,
@@ -769,12 +1037,12 @@ static void launch_fprj(const Geom *g, float *sino, const float *img, const int3
                                                    rho_nb, bt);
       b0 += 4;
     } else if (rem >= 2 && wmax >= 2) {
-      k_fprj<MODE, 2><<<nprj, FP_THREADS, 0, st>>>(view(g), img, sino, subs, prompts, rho,
-                                                   rho_nb, bt);
+      k_fprj_g<MODE, 2, 3><<<nprj, fpg_threads<3>(), 0, st>>>(view(g), img, sino, subs, prompts,
+                                                              rho, rho_nb, bt);
       b0 += 2;
     } else {
-      k_fprj<MODE, 1><<<nprj, FP_THREADS, 0, st>>>(view(g), img, sino, subs, prompts, rho,
-                                                   rho_nb, bt);
+      k_fprj_g<MODE, 1, FP_G1><<<nprj, fpg_threads<FP_G1>(), 0, st>>>(view(g), img, sino, subs,
+                                                                      prompts, rho, rho_nb, bt);
       b0 += 1;
     }
   }
