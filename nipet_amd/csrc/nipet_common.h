@@ -46,13 +46,14 @@ struct Geom {
   int16_t *dr_li = nullptr;    // [64 * 64] li of (d, r0) at d * 64 + r0, -1 if absent
   int32_t *sn_off = nullptr;   // [nsn + 1] CSR of the span-1 LORs of each output sinogram:
   int16_t *sn_mem = nullptr;   // [2 nli] 2 li (LOR a) or 2 li + 1 (mirror b)
-  // forward-projector lane slots: slot s walks the LORs (d, r0 + k), k < G,
-  // of one ring difference; fp_slot[i][s] = d * 64 + r0 for G = FP_GROUPS[i]
-  int16_t *fp_slot[2] = {nullptr, nullptr};
-  int fp_nslot[2] = {0, 0};
+  // forward-projector lane slots: slot s walks the LORs (d, r0 + k), k < FP_G,
+  // of one ring difference; fp_slot[s] = d * 64 + r0
+  int16_t *fp_slot = nullptr;
+  int fp_nslot = 0;
 };
 
-constexpr int FP_GROUPS[2] = {3, 5};
+constexpr int FP_G = 3;          // LORs per forward-projector lane
+constexpr int FP_G_SLOTS = 768;  // lanes of that kernel
 
 void set_error(const char *fmt, ...);
 int hip_fail(hipError_t e, const char *what);

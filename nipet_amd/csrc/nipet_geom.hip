@@ -283,8 +283,7 @@ static void geom_free(Geom *g) {
   hipFree(g->dr_li);
   hipFree(g->sn_off);
   hipFree(g->sn_mem);
-  hipFree(g->fp_slot[0]);
-  hipFree(g->fp_slot[1]);
+  hipFree(g->fp_slot);
 }
 
 static int geom_build(const NipetGeomDesc *d, NipetGeom *g) {
@@ -366,26 +365,25 @@ static int geom_build(const NipetGeomDesc *d, NipetGeom *g) {
       if (liz[li].y > 0.f) snmem[fill[lisn[li].y]++] = (int16_t)(2 * li + 1);
     }
   }
-  // forward-projector slots: the LORs of one ring difference d in groups of G
-  // consecutive r0 (ordered by d, then r0); LOR (d, r0 + k) lies exactly 2k
-  // planes above (d, r0) along the whole ray
-  for (int i = 0; i < 2; i++) {
-    const int G = FP_GROUPS[i];
+  // forward-projector slots: the LORs of one ring difference d in groups of
+  // FP_G consecutive r0 (ordered by d, then r0); LOR (d, r0 + k) lies exactly
+  // 2k planes above (d, r0) along the whole ray
+  {
     std::vector<int16_t> slots;
     for (int dd = 0; dd <= g->dmax; dd++)
-      for (int r0 = 0; r0 + dd <= 63; r0 += G) {
+      for (int r0 = 0; r0 + dd <= 63; r0 += FP_G) {
         bool any = false;
-        for (int k = 0; k < G && r0 + k + dd <= 63; k++) any |= drli[dd * 64 + r0 + k] >= 0;
+        for (int k = 0; k < FP_G && r0 + k + dd <= 63; k++) any |= drli[dd * 64 + r0 + k] >= 0;
         if (any) slots.push_back((int16_t)(dd * 64 + r0));
       }
-    g->fp_nslot[i] = (int)slots.size();
-    if (g->fp_nslot[i] > (G == 3 ? 768 : 448)) {
-      set_error("%d forward-projector slots of %d LORs exceed the kernel's lanes",
-                g->fp_nslot[i], G);
+    g->fp_nslot = (int)slots.size();
+    if (g->fp_nslot > FP_G_SLOTS) {
+      set_error("%d forward-projector slots exceed the kernel's %d lanes", g->fp_nslot,
+                FP_G_SLOTS);
       return NIPET_EINVAL;
     }
-    NIPET_HIP(hipMalloc(&g->fp_slot[i], slots.size() * sizeof(int16_t)));
-    NIPET_HIP(hipMemcpy(g->fp_slot[i], slots.data(), slots.size() * sizeof(int16_t),
+    NIPET_HIP(hipMalloc(&g->fp_slot, slots.size() * sizeof(int16_t)));
+    NIPET_HIP(hipMemcpy(g->fp_slot, slots.data(), slots.size() * sizeof(int16_t),
                         hipMemcpyHostToDevice));
   }
   NIPET_HIP(hipMalloc(&g->dr_li, drli.size() * sizeof(int16_t)));

@@ -44,8 +44,8 @@ struct GeomView {
   const int16_t *dr_li;
   const int32_t *sn_off;
   const int16_t *sn_mem;
-  const int16_t *fp_slot[2];
-  int fp_nslot[2];
+  const int16_t *fp_slot;
+  int fp_nslot;
   int naw, nli, ndirect, nsn, dmax;
   float vox_z;
 };
@@ -62,10 +62,8 @@ static GeomView view(const Geom *g) {
   v.dr_li = g->dr_li;
   v.sn_off = g->sn_off;
   v.sn_mem = g->sn_mem;
-  for (int i = 0; i < 2; i++) {
-    v.fp_slot[i] = g->fp_slot[i];
-    v.fp_nslot[i] = g->fp_nslot[i];
-  }
+  v.fp_slot = g->fp_slot;
+  v.fp_nslot = g->fp_nslot;
   v.naw = g->naw;
   v.nli = g->nli;
   v.ndirect = g->ndirect;
@@ -412,11 +410,14 @@ __global__ __launch_bounds__(FP_THREADS, NB >= 4 ? 4 : 6) void k_fprj(GeomView g
 // arithmetic equals k_fprj's, so both kernels give the same values. Slots (d, r0) come from the
 // geometry (fp_slot); lanes whose group runs past ring 63 walk phantom LORs
 // whose planes lie in zero guard planes and are never written out.
-// G is odd: lanes of one ring difference then sit 2G planes apart, which puts
-// them on distinct LDS banks (plane stride 17 floats; an even G would fold
-// them onto 8 or 16 banks).
+// G = 3 (odd): lanes of one ring difference then sit 6 planes apart, which
+// puts them on distinct LDS banks (plane stride 17 floats; an even G would fold
+// them onto 8 or 16 banks). Measured on MI355X: G = 3 1.19 ms per subset,
+// G = 5 1.31 ms, G = 4 3.4 ms (bank conflicts), one LOR per lane 1.71 ms. With
+// two replicates (plane stride 34 floats) every G conflicts and k_fprj<., 2>
+// is faster, so only single images use this kernel.
 template <int G>
-constexpr int fpg_threads() { return G == 3 ? 768 : 448; }  // >= 712 (G = 3) / 439 (G = 5) slots
+constexpr int fpg_threads() { return FP_G_SLOTS; }  // >= 712 slots of 3 LORs
 template <int G>
 constexpr int fpg_rounds() { return 1; }
 template <int G>
@@ -523,7 +524,7 @@ __global__ __launch_bounds__(fpg_threads<G>()) void k_fprj_g(GeomView g, const f
                                                            int rho_nb, Batch bt) {
   constexpr int CH = fp_chunk<NB>(), PS = fp_ps<NB>();
   constexpr int R = fpg_rounds<G>(), NP = fpg_planes<G>(), FPG_THREADS = fpg_threads<G>();
-  constexpr int GI = G == FP_GROUPS[0] ? 0 : 1;
+  static_assert(G == FP_G, "slot tables are built for FP_G LORs per lane");
   constexpr int ZN = NP * PS * NB, LN = 2 * LI_MAX;
   __shared__ __align__(16) float lds[ZN > LN ? ZN : LN];
   __shared__ float2 sr[CH];
@@ -533,8 +534,8 @@ __global__ __launch_bounds__(fpg_threads<G>()) void k_fprj_g(GeomView g, const f
   const int row = blockIdx.x;
   const int aw = subs ? subs[row] : row;
   const int nsn = g.nsn, nbt = bt.nbt, b0 = bt.b0;
-  const int nslot = g.fp_nslot[GI];
-  const int16_t *slots = g.fp_slot[GI];
+  const int nslot = g.fp_nslot;
+  const int16_t *slots = g.fp_slot;
   float *out = sino + (size_t)row * nsn * nbt + b0;
   if (aw < 0 || aw >= g.naw) {  // invalid bin index: zero row
     for (int i = tid; i < nsn * NB; i += FPG_THREADS) out[(i / NB) * nbt + i % NB] = 0.f;
@@ -1022,7 +1023,6 @@ using namespace nipet;
 // Largest replicate sub-batch per kernel instance (4, 2 or 1), set by the
 // autotuner through nipet_tune_set; process-wide.
 static std::atomic<int> g_width_fp{4}, g_width_bp{4};
-constexpr int FP_G1 = 3;  // LORs per lane of the single-image forward projector
 
 template <int MODE>
 static void launch_fprj(const Geom *g, float *sino, const float *img, const int32_t *subs,
@@ -1037,12 +1037,12 @@ static void launch_fprj(const Geom *g, float *sino, const float *img, const int3
                                                    rho_nb, bt);
       b0 += 4;
     } else if (rem >= 2 && wmax >= 2) {
-      k_fprj_g<MODE, 2, 3><<<nprj, fpg_threads<3>(), 0, st>>>(view(g), img, sino, subs, prompts,
-                                                              rho, rho_nb, bt);
+      k_fprj<MODE, 2><<<nprj, FP_THREADS, 0, st>>>(view(g), img, sino, subs, prompts, rho,
+                                                   rho_nb, bt);
       b0 += 2;
     } else {
-      k_fprj_g<MODE, 1, FP_G1><<<nprj, fpg_threads<FP_G1>(), 0, st>>>(view(g), img, sino, subs,
-                                                                      prompts, rho, rho_nb, bt);
+      k_fprj_g<MODE, 1, FP_G><<<nprj, fpg_threads<FP_G>(), 0, st>>>(view(g), img, sino, subs,
+                                                                    prompts, rho, rho_nb, bt);
       b0 += 1;
     }
   }
