@@ -685,7 +685,8 @@ __global__ __launch_bounds__(fpg_threads<G>()) void k_fprj_g(GeomView g, const f
 // point at a zero row, so clamped indices read zeros. No atomics inside a ray:
 // each wave flushes its segments through a private LDS buffer with one
 // coalesced global float atomic per (segment, plane, replicate).
-__device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+// min(max(v, lo), hi): one v_med3_i32
+__device__ inline int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
 template <int NB>
 constexpr int bp_spw() { return NB == 1 ? 8 : 4; }
@@ -967,20 +968,6 @@ __global__ void k_osem_update(float *img, const float *b, const float *sens, con
   }
 }
 
-// Siemens (nsn, 252*344) -> device (naw, nsn) gather through aw2ali
-// (d_remgaps, src/scanner_0.cu:94-105); input float32 (dtype 0) or uint16 (1).
-template <typename T>
-__global__ void k_remgaps(float *out, const T *in, const int32_t *aw2ali, int naw, int nsn,
-                          int nbinang) {
-  int64_t n = (int64_t)naw * nsn;
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) {
-    int aw = (int)(i / nsn), s = (int)(i - (int64_t)aw * nsn);
-    out[i] = (float)in[(int64_t)s * nbinang + aw2ali[aw]];
-  }
-}
-
 // Siemens histogram with replicates innermost, in[(s * nbinang + ali) * nrep + r]
 // (uint32 counts) -> device prompts out[(aw * nsn + s) * nb + b] (uint16,
 // saturating at 65535) for replicates r = r0 + b, b < nb.
@@ -997,16 +984,68 @@ __global__ void k_remgaps_u16b(uint16_t *out, const uint32_t *in, const int32_t 
   }
 }
 
-// device (naw, nsn) -> Siemens (nsn, 252*344) scatter, gaps stay zero
-// (d_putgaps, src/scanner_0.cu:158-166, plus the transpose of mmraux.py:1141).
-__global__ void k_putgaps(float *out, const float *in, const int32_t *aw2ali, int naw, int nsn,
-                          int nbinang) {
-  int64_t n = (int64_t)naw * nsn;
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) {
-    int aw = (int)(i / nsn), s = (int)(i - (int64_t)aw * nsn);
-    out[(int64_t)s * nbinang + aw2ali[aw]] = in[i];
+// Gap removal / insertion (d_remgaps / d_putgaps, src/scanner_0.cu:94-105,
+// 158-166, plus the transpose of mmraux.py:1141): tiled transposes between the
+// Siemens layout (nsn, nbinang) and the device layout (naw, nsn): a 64 x 64 tile of (Siemens bin, sinogram) passes through
+// LDS, so Siemens rows are read / written along the bin index and device rows
+// along the sinogram index, both coalesced. inv[ali] = active bin of Siemens
+// bin ali, or -1 in the gaps (zeros on output).
+constexpr int GT = 64;
+
+__global__ void k_gap_inv(int32_t *inv, int nbinang) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nbinang) inv[i] = -1;
+}
+
+__global__ void k_gap_inv_set(int32_t *inv, const int32_t *aw2ali, int naw, int nbinang) {
+  const int aw = blockIdx.x * blockDim.x + threadIdx.x;
+  if (aw < naw) {
+    const int a = aw2ali[aw];
+    if (a >= 0 && a < nbinang) inv[a] = aw;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_remgaps_t(float *__restrict__ out,
+                                                   const T *__restrict__ in,
+                                                   const int32_t *__restrict__ inv, int nsn,
+                                                   int nbinang) {
+  __shared__ float tile[GT][GT + 1];
+  const int a0 = blockIdx.x * GT, s0 = blockIdx.y * GT;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int r = ty; r < GT; r += 4) {  // r: sinogram, tx: Siemens bin
+    const int s = s0 + r, a = a0 + tx;
+    tile[r][tx] = (s < nsn && a < nbinang) ? (float)in[(int64_t)s * nbinang + a] : 0.f;
+  }
+  __syncthreads();
+  for (int r = ty; r < GT; r += 4) {  // r: Siemens bin, tx: sinogram
+    const int a = a0 + r, s = s0 + tx;
+    if (a >= nbinang || s >= nsn) continue;
+    const int aw = inv[a];
+    if (aw >= 0) out[(int64_t)aw * nsn + s] = tile[tx][r];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_putgaps_t(float *__restrict__ out,
+                                                   const float *__restrict__ in,
+                                                   const int32_t *__restrict__ inv, int nsn,
+                                                   int nbinang) {
+  __shared__ float tile[GT][GT + 1];
+  const int a0 = blockIdx.x * GT, s0 = blockIdx.y * GT;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int r = ty; r < GT; r += 4) {  // r: Siemens bin, tx: sinogram
+    const int a = a0 + r, s = s0 + tx;
+    float v = 0.f;
+    if (a < nbinang && s < nsn) {
+      const int aw = inv[a];
+      if (aw >= 0) v = in[(int64_t)aw * nsn + s];
+    }
+    tile[tx][r] = v;
+  }
+  __syncthreads();
+  for (int r = ty; r < GT; r += 4) {  // r: sinogram, tx: Siemens bin (gaps get 0)
+    const int s = s0 + r, a = a0 + tx;
+    if (s < nsn && a < nbinang) out[(int64_t)s * nbinang + a] = tile[r][tx];
   }
 }
 
@@ -1155,15 +1194,18 @@ int nipet_remgaps(float *out, const void *in, int in_dtype, const int32_t *aw2al
     set_error("nipet_remgaps: bad argument");
     return NIPET_EINVAL;
   }
-  int64_t n = (int64_t)naw * nsn;
   hipStream_t st = (hipStream_t)stream;
+  int32_t *inv = nullptr;
+  NIPET_HIP(hipMallocAsync((void **)&inv, (size_t)nbinang * sizeof(int32_t), st));
+  k_gap_inv<<<(nbinang + 255) / 256, 256, 0, st>>>(inv, nbinang);
+  k_gap_inv_set<<<(naw + 255) / 256, 256, 0, st>>>(inv, aw2ali, naw, nbinang);
+  const dim3 grid((nbinang + GT - 1) / GT, (nsn + GT - 1) / GT);
   if (in_dtype == 0)
-    k_remgaps<float><<<ew_blocks(n), 256, 0, st>>>(out, (const float *)in, aw2ali, naw, nsn,
-                                                   nbinang);
+    k_remgaps_t<float><<<grid, 256, 0, st>>>(out, (const float *)in, inv, nsn, nbinang);
   else
-    k_remgaps<uint16_t><<<ew_blocks(n), 256, 0, st>>>(out, (const uint16_t *)in, aw2ali, naw,
-                                                      nsn, nbinang);
+    k_remgaps_t<uint16_t><<<grid, 256, 0, st>>>(out, (const uint16_t *)in, inv, nsn, nbinang);
   NIPET_HIP(hipGetLastError());
+  NIPET_HIP(hipFreeAsync(inv, st));
   return NIPET_OK;
 }
 
@@ -1207,11 +1249,15 @@ int nipet_putgaps(float *out, const float *in, const int32_t *aw2ali, int naw, i
     set_error("nipet_putgaps: bad argument");
     return NIPET_EINVAL;
   }
-  int64_t n = (int64_t)naw * nsn, nout = (int64_t)nsn * nbinang;
   hipStream_t st = (hipStream_t)stream;
-  k_fill<<<ew_blocks(nout), 256, 0, st>>>(out, 0.f, nout);
-  k_putgaps<<<ew_blocks(n), 256, 0, st>>>(out, in, aw2ali, naw, nsn, nbinang);
+  int32_t *inv = nullptr;
+  NIPET_HIP(hipMallocAsync((void **)&inv, (size_t)nbinang * sizeof(int32_t), st));
+  k_gap_inv<<<(nbinang + 255) / 256, 256, 0, st>>>(inv, nbinang);
+  k_gap_inv_set<<<(naw + 255) / 256, 256, 0, st>>>(inv, aw2ali, naw, nbinang);
+  const dim3 grid((nbinang + GT - 1) / GT, (nsn + GT - 1) / GT);
+  k_putgaps_t<<<grid, 256, 0, st>>>(out, in, inv, nsn, nbinang);
   NIPET_HIP(hipGetLastError());
+  NIPET_HIP(hipFreeAsync(inv, st));
   return NIPET_OK;
 }
 
