@@ -19,7 +19,27 @@ __all__ = [
     'get_mmrparams', 'get_mmr_constants', 'mmrinit', 'transaxial_lut', 'axial_lut',
     'span1_span11', 'putgaps', 'remgaps', 'convert2e7', 'convert2dev', 'get_cylinder',
     'frwd_prj', 'back_prj', 'trnx_prj', 'get_subsets14', 'psf_config', 'osemone', 'mmrhist',
-    'lminfo']
+    'lminfo', 'dev_info']
+
+
+def dev_info(showprop=False):
+    """HIP devices visible to this process as a list of (name, total memory
+    [MiB], gfx architecture, compute units), like ``niftypet.nipet.dev_info``
+    (``ninst.dinf.dev_info``); prints the properties when ``showprop``."""
+    try:
+        import torch
+        n = torch.cuda.device_count() if torch.cuda.is_available() else 0
+    except Exception:
+        n = 0
+    out = []
+    for i in range(n):
+        pr = torch.cuda.get_device_properties(i)
+        out.append((pr.name, pr.total_memory >> 20, getattr(pr, 'gcnArchName', ''),
+                    pr.multi_processor_count))
+        if showprop:
+            print(f'device {i}: {pr.name}, {pr.total_memory >> 20} MiB, '
+                  f'{getattr(pr, "gcnArchName", "")}, {pr.multi_processor_count} CUs')
+    return out
 
 
 def __getattr__(name):

@@ -47,3 +47,16 @@ def test_status_codes_without_gpu(L):
     assert L.nipet_version().startswith(b'nipet_amd')
     with pytest.raises(_lib.NipetError):
         _lib.check(-1, 'x')
+
+
+def test_dev_info_quiet(capsys):
+    """dev_info lists devices without printing unless asked (the reference's
+    tests/test_basic.py:6-12)."""
+    import nipet_amd as nip
+    devs = nip.dev_info()
+    out, err = capsys.readouterr()
+    assert not out and not err
+    nip.dev_info(showprop=True)
+    out, err = capsys.readouterr()
+    assert not err
+    assert not devs or out
