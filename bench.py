@@ -21,7 +21,9 @@ a n FP(phantom), 15 % delayeds) is histogrammed once into 64 Poisson(1)
 bootstrap replicates and reconstructed 4x14 as one batch (one HIP graph
 launch, autotuned sub-batch widths); it is compared with plain
 reconstructions of single replicates (one histogram session with that
-replicate, osem_prepare, osem_run), whose images it must equal.
+replicate, osem_prepare, osem_run), whose images it must equal. The same
+stream is cut into 24 dynamic frames of 10-100 s (extra field "frames"),
+reconstructed as one batch against plain per-frame reconstructions.
 
 Corrections (extra field "corrections", skipped by --no-corr): randoms from
 fan-sums, the normalisation sinogram from components and one PSF convolution
@@ -121,21 +123,82 @@ def copy_calibration(nbytes):
           flush=True)
 
 
-def batched_bootstrap(params, d, nrep, nwords, nplain=2):
-    """Batched vs plain bootstrap reconstructions (see the module docstring)."""
+LM_SECONDS = 600
+
+
+def lm_stream(params, d, nwords):
+    """Synthetic 600 s list-mode stream on the device whose prompts follow
+    a n FP(phantom)."""
     import torch
 
     from nipet_amd.lm.lmsim import synth_lm
+    from nipet_amd.prj.mmrprj import frwd_prj
+    per_ms = max(1, int(round(nwords / (LM_SECONDS * 1000))))
+    emis = frwd_prj(torch.from_numpy(d['img']).cuda(), params, dev_out=True)
+    log(f'synthesising {LM_SECONDS} s list-mode stream ({per_ms} events/ms)')
+    words = synth_lm(params, LM_SECONDS, per_ms, weights=d['attn'] * d['norm'] * emis, seed=5)
+    del emis
+    torch.cuda.synchronize()
+    return words
+
+
+# 24 dynamic frames of uneven length covering the 600 s stream
+FRAMES = ([(10 * i, 10 * i + 10) for i in range(10)] +
+          [(100 + 20 * i, 120 + 20 * i) for i in range(8)] +
+          [(260 + 35 * i, 295 + 35 * i) for i in range(4)] +
+          [(400, 500), (500, 600)])
+
+
+def batched_frames(params, d, words, nplain=2):
+    """24 dynamic frames (10-100 s) reconstructed 4x14 in one batch (one graph
+    launch, autotuned widths) vs plain per-frame reconstructions."""
+    import torch
+
+    from nipet_amd.prj.batch import frame_prompts, frames_osem
+    from nipet_amd.prj.osem import osem_prepare, osem_run
+    rs = (d['randoms'] + d['scatter']) * 1e-3
+    rho = torch.stack([rs * ((t1 - t0) / LM_SECONDS) for t0, t1 in FRAMES], dim=2).contiguous()
+    torch.cuda.synchronize()
+    log(f'batched frames: {len(FRAMES)}')
+    t0 = time.perf_counter()
+    imgs, info = frames_osem(words, params, FRAMES, d['attn'], d['norm'], rho, itr=NITER)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    t_plain, rels = [], []
+    for k in range(nplain):
+        f = (k * 13) % len(FRAMES)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        p, _ = frame_prompts(words, params, [FRAMES[f]])
+        st = osem_prepare(params, p[..., 0].contiguous(), d['attn'], d['norm'],
+                          rho[..., f].contiguous())
+        img = osem_run(st, NITER, check_counts=False)
+        torch.cuda.synchronize()
+        t_plain.append(time.perf_counter() - t0)
+        rels.append(float(torch.linalg.vector_norm((imgs[..., f] - img).double()) /
+                          torch.linalg.vector_norm(img.double())))
+        del st, img, p
+    tp = sum(t_plain) / len(t_plain)
+    nf = len(FRAMES)
+    log(f'frames: batched {t_batch:.2f} s for {nf}, plain {tp:.3f} s per frame')
+    return {'workload': f'{nf} dynamic frames of 10-100 s of the {LM_SECONDS} s stream, OSEM '
+                        f'{NITER}x{NSUB} each, span-11, per-frame randoms+scatter',
+            'recons_per_s': round(nf / t_batch, 4), 'plain_recons_per_s': round(1.0 / tp, 4),
+            'speedup_vs_plain': round((nf / t_batch) * tp, 3), 'batch_s': round(t_batch, 3),
+            'plain_checked': nplain, 'max_rel_l2_vs_plain': max(rels),
+            'widths': {'fprj': info['tune']['fprj'], 'bprj': info['tune']['bprj']},
+            'timing': 'wall clock incl. per-frame histogramming, corrections, sensitivity, '
+                      'autotune, graph capture'}
+
+
+def batched_bootstrap(params, d, words, nrep, nplain=2):
+    """Batched vs plain bootstrap reconstructions (see the module docstring)."""
+    import torch
+
     from nipet_amd.lm.mmrhist import hist_words
     from nipet_amd.prj.batch import _remgaps_b, bootstrap_osem
-    from nipet_amd.prj.mmrprj import frwd_prj
     from nipet_amd.prj.osem import osem_prepare, osem_run
-    dur = 600
-    per_ms = max(1, int(round(nwords / (dur * 1000))))
-    emis = frwd_prj(torch.from_numpy(d['img']).cuda(), params, dev_out=True)
-    log(f'synthesising {dur} s list-mode stream ({per_ms} events/ms)')
-    words = synth_lm(params, dur, per_ms, weights=d['attn'] * d['norm'] * emis, seed=5)
-    del emis
+    dur = LM_SECONDS
     rs = (d['randoms'] + d['scatter']) * 1e-3
     seed = 2024
     torch.cuda.synchronize()
@@ -320,7 +383,10 @@ def main():
         log('corrections')
         out['corrections'] = corrections(params, d)
     if not args.no_batch:
-        out['batched'] = batched_bootstrap(params, d, args.nrep, args.lm_words)
+        words = lm_stream(params, d, args.lm_words)
+        out['batched'] = batched_bootstrap(params, d, words, args.nrep)
+        out['frames'] = batched_frames(params, d, words)
+        del words
     if not args.no_cpu:
         log('CPU baseline')
         out['cpu_baseline'] = cpu_baseline(params, args.cpu_seconds)
