@@ -689,7 +689,7 @@ __global__ __launch_bounds__(fpg_threads<G>()) void k_fprj_g(GeomView g, const f
 __device__ inline int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
 template <int NB>
-constexpr int bp_spw() { return NB == 1 ? 8 : 4; }
+constexpr int bp_spw() { return NB == 1 ? 8 : (NB == 2 ? 4 : 6); }  // measured per width
 
 template <int NB>
 __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g, float *__restrict__ img,
