@@ -23,7 +23,9 @@ launch, autotuned sub-batch widths); it is compared with plain
 reconstructions of single replicates (one histogram session with that
 replicate, osem_prepare, osem_run), whose images it must equal. The same
 stream is cut into 24 dynamic frames of 10-100 s (extra field "frames"),
-reconstructed as one batch against plain per-frame reconstructions.
+reconstructed as one batch against plain per-frame reconstructions; the
+"histogram" field times the histogramming of the stream alone (plain and with
+64 bootstrap replicates).
 
 Corrections (extra field "corrections", skipped by --no-corr): randoms from
 fan-sums, the normalisation sinogram from components and one PSF convolution
@@ -140,6 +142,30 @@ def lm_stream(params, d, nwords):
     del emis
     torch.cuda.synchronize()
     return words
+
+
+def histogram(params, words, nrep=64):
+    """List-mode histogramming of the 600 s stream (6e8 words): plain (span-11
+    prompts / delayeds, head curves, fan-sums, buckets, SSRB) and with nrep
+    Poisson(1) bootstrap replicates, device-resident words, wall clock."""
+    import torch
+
+    from nipet_amd.lm.mmrhist import hist_words
+    out = {}
+    for n, btp in ((1, 0), (nrep, 2)):
+        hist_words(words[:1 << 20], params, btp=btp, nrep=n)  # load the kernels
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        h = hist_words(words, params, btp=btp, seed=7, nrep=n)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        key = 'plain' if n == 1 else f'bootstrap_{n}'
+        out[key] = {'s': round(dt, 4), 'words_per_s': round(words.numel() / dt, 1),
+                    'prompts': int(h.out['phc'][:, 0].sum(dtype=torch.int64))}
+        del h
+    out['workload'] = (f'{words.numel()} words over {LM_SECONDS} s (1 ms tags, 85 % prompts), '
+                       'span-11; wall clock incl. session set-up and output zeroing')
+    return out
 
 
 # 24 dynamic frames of uneven length covering the 600 s stream
@@ -384,6 +410,7 @@ def main():
         out['corrections'] = corrections(params, d)
     if not args.no_batch:
         words = lm_stream(params, d, args.lm_words)
+        out['histogram'] = histogram(params, words, args.nrep)
         out['batched'] = batched_bootstrap(params, d, words, args.nrep)
         out['frames'] = batched_frames(params, d, words)
         del words
