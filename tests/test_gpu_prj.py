@@ -145,3 +145,40 @@ def test_operator_shapes(torch, params, img):
     assert bim.shape == (127, 344, 344)
     with pytest.raises(ValueError):
         nip.frwd_prj(np.zeros((10, 10, 10), np.float32), params)
+
+
+@pytest.fixture(scope='module')
+def params1(params):
+    """The same scanner in span 1 (4084 sinograms, one LOR per bin)."""
+    Cnt = dict(params['Cnt'])
+    Cnt['SPN'] = 1
+    return {'Cnt': Cnt, 'txLUT': params['txLUT'], 'axLUT': params['axLUT']}
+
+
+def test_span1_projectors_vs_oracle(torch, params1, subsets, img):
+    """Span-1 geometry (li2sn1): emission and attenuation FP within 1e-5, BP
+    within 1e-5 of the image maximum, adjoint identity to 1e-6."""
+    from oracle import oracle as O
+
+    import nipet_amd as nip
+    sub = subsets[3]
+    gpu = nip.frwd_prj(img, params1, isub=sub, dev_out=True, fullsino_out=False)
+    assert gpu.shape == (len(sub), 4084)
+    pick = np.random.default_rng(11).choice(len(sub), 96, replace=False)
+    assert _rel(gpu[pick], O.fprj(params1, img, sub[pick])) < FP_RTOL
+    mu = 0.096 * (phantom(2, nsph=0) > 0).astype(np.float32)
+    att = nip.frwd_prj(mu, params1, isub=sub, dev_out=True, fullsino_out=False, attenuation=True)
+    assert _rel(att[pick], O.fprj(params1, mu, sub[pick], att=True)) < FP_RTOL
+    rows = _rows(sub, 96, seed=12)
+    y = np.random.default_rng(13).random((len(rows), 4084)).astype(np.float32)
+    bimg = nip.back_prj(y, params1, isub=rows, dev_out=True)
+    ref = O.bprj(params1, y.astype(np.float64), rows)
+    assert np.abs(bimg.astype(np.float64) - ref).max() / np.abs(ref).max() < 1e-5
+    x = torch.from_numpy(img).cuda()
+    yt = torch.rand((len(sub), 4084), device='cuda',
+                    generator=torch.Generator('cuda').manual_seed(14))
+    ax = nip.frwd_prj(x, params1, isub=sub, dev_out=True, fullsino_out=False)
+    aty = nip.back_prj(yt, params1, isub=sub, dev_out=True)
+    a = float((ax.double() * yt.double()).sum())
+    b = float((x.double() * aty.double()).sum())
+    assert abs(a - b) / abs(a) < 1e-6
