@@ -27,6 +27,10 @@ reconstructed as one batch against plain per-frame reconstructions; the
 "histogram" field times the histogramming of the stream alone (plain and with
 64 bootstrap replicates).
 
+Autograd (extra field "autograd", --map-steps 0 skips): 200 gradient MAP steps
+of the Poisson likelihood + quadratic prior through the differentiable
+projectors, one checkpoint per subset.
+
 Corrections (extra field "corrections", skipped by --no-corr): randoms from
 fan-sums, the normalisation sinogram from components and one PSF convolution
 of the phantom, each timed with HIP events.
@@ -61,6 +65,8 @@ def parse():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-batch', action='store_true', help='skip the batched bootstrap leg')
+    ap.add_argument('--map-steps', type=int, default=200,
+                    help='gradient MAP steps of the autograd leg (0: skip)')
     ap.add_argument('--no-corr', action='store_true',
                     help='skip the randoms / normalisation / PSF timing leg')
     ap.add_argument('--nrep', type=int, default=64, help='bootstrap replicates of the batch')
@@ -166,6 +172,32 @@ def histogram(params, words, nrep=64):
     out['workload'] = (f'{words.numel()} words over {LM_SECONDS} s (1 ms tags, 85 % prompts), '
                        'span-11; wall clock incl. session set-up and output zeroing')
     return out
+
+
+def map_leg(params, d, steps):
+    """Gradient MAP through the differentiable projectors (torch.autograd):
+    `steps` preconditioned gradient steps of the Poisson NLL over all 14
+    subsets + quadratic prior (beta 1e-3), one checkpoint per subset."""
+    import torch
+
+    import nipet_amd as nip
+    from nipet_amd.prj.autograd import map_recon
+    subs = [nip.get_subsets14(k, params)[0] for k in range(NSUB)]
+    dev = lambda a: a if isinstance(a, torch.Tensor) else torch.from_numpy(a).cuda()  # noqa: E731
+    prompts = dev(d['prompts']).to(torch.int32).to(torch.float32)
+    mu = dev(d['mu'])
+    rs = dev(d['randoms']) + dev(d['scatter'])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, hist = map_recon(params, prompts, dev(d['norm']), rs, mu, subs, beta=1e-3, steps=steps)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    log(f'MAP: {steps} steps in {dt:.2f} s')
+    return {'workload': f'{steps} gradient MAP steps (Poisson NLL over 14 subsets + quadratic '
+                        'prior beta 1e-3, checkpoint per subset), 320x320x127 image',
+            'steps': steps, 's': round(dt, 3), 'ms_per_step': round(1e3 * dt / steps, 2),
+            'objective_first_last': [float(hist[0]), float(hist[-1])],
+            'timing': 'wall clock incl. attenuation FP and sensitivity BP of the set-up'}
 
 
 # 24 dynamic frames of uneven length covering the 600 s stream
@@ -414,6 +446,8 @@ def main():
         out['batched'] = batched_bootstrap(params, d, words, args.nrep)
         out['frames'] = batched_frames(params, d, words)
         del words
+    if args.map_steps > 0:
+        out['autograd'] = map_leg(params, d, args.map_steps)
     if not args.no_cpu:
         log('CPU baseline')
         out['cpu_baseline'] = cpu_baseline(params, args.cpu_seconds)

@@ -244,7 +244,7 @@ def map_recon(scanner_params, prompts, norm, rs, mu, subsets, beta=1e-3, steps=2
         for s in range(len(prjs)):
             phi = phi + (ckpt(nll_s, xv, s, use_reentrant=False) if checkpoint else nll_s(xv, s))
         g, = torch.autograd.grad(phi, xv)
-        hist.append(float(phi))
+        hist.append(float(phi.detach()))
         with torch.no_grad():
             x = torch.clamp_min(x - step * x * pre_scale * g, 0.0)
             x = torch.where(mask, x, torch.zeros_like(x))
