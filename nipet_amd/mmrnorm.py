@@ -85,7 +85,10 @@ def get_components(datain, Cnt, auxdata=None):
     f0 = nhdr.find('scanner quantification factor')
     f1 = f0 + nhdr[f0:].find('\n')
     p = re.compile(r'(?<=:=)\s*\d{1,5}[.]\d{3,10}[e][+-]\d{1,4}')
-    qf = float(p.findall(nhdr[f0:f1] if f1 > f0 else nhdr[f0:])[0])
+    found = p.findall(nhdr[f0:f1] if f1 > f0 else nhdr[f0:])
+    if not found:
+        raise ValueError(f'{fhdr}: unreadable scanner quantification factor')
+    qf = float(found[0])
     return {'qf': qf, 'qf_loc': QF_LOC, 'geo': geo, 'cinf': cinf, 'ceff': ceff, 'axe1': axe1,
             'dtp': dtp, 'dtnp': dtnp, 'dtc': dtc, 'axe2': axe2, 'axf1': axf1,
             'sax_f11': sax_f11, 'sax_f1': sax_f1}, nhdr
