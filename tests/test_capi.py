@@ -60,3 +60,40 @@ def test_dev_info_quiet(capsys):
     out, err = capsys.readouterr()
     assert not err
     assert not devs or out
+
+
+def test_new_entry_points_reject_bad_arguments(L, params):
+    """Randoms, normalisation, online variance and PSF entry points validate
+    their descriptions before touching the device (status codes, messages)."""
+    import numpy as np
+
+    from nipet_amd import _lib
+    tx, ax = params['txLUT'], params['axLUT']
+    h = ct.c_void_p()
+    assert L.nipet_rnd_create(None, ct.byref(h)) == -1
+    keep = [np.ascontiguousarray(tx['cij'], np.int8), np.ascontiguousarray(tx['crsri'], np.int16),
+            np.ascontiguousarray(tx['s2cr'], np.int16), np.ascontiguousarray(ax['sn1_rno'], np.int16),
+            np.ascontiguousarray(ax['sn1_sn11'], np.int16)]
+    d = _lib.NipetRndDesc(tx['Naw'], 500, 4084, 60, *[k.ctypes.data for k in keep])  # bad nsn
+    assert L.nipet_rnd_create(ct.byref(d), ct.byref(h)) == -1
+    assert b'span 1 or 11' in L.nipet_last_error()
+    bad_cij = np.zeros((448, 448), np.int8)                                          # no runs
+    d = _lib.NipetRndDesc(tx['Naw'], 837, 4084, 60, bad_cij.ctypes.data,
+                          *[k.ctypes.data for k in keep[1:]])
+    assert L.nipet_rnd_create(ct.byref(d), ct.byref(h)) == -1
+    assert b'circular run' in L.nipet_last_error()
+    assert L.nipet_norm_create(None, ct.byref(h)) == -1
+    nk = [np.ascontiguousarray(tx['aw2ali'], np.int32), np.ascontiguousarray(ax['sn1_rno'], np.int16),
+          np.ascontiguousarray(ax['sn1_sn11'], np.int16),
+          np.ascontiguousarray(ax['sn1_sn11no'], np.int8)]
+    nd = _lib.NipetNormDesc(tx['Naw'], 3, *[k.ctypes.data for k in nk])               # bad span
+    assert L.nipet_norm_create(ct.byref(nd), ct.byref(h)) == -1
+    bad_ali = nk[0].copy()
+    bad_ali[5] = 252 * 344
+    nd = _lib.NipetNormDesc(tx['Naw'], 11, bad_ali.ctypes.data, *[k.ctypes.data for k in nk[1:]])
+    assert L.nipet_norm_create(ct.byref(nd), ct.byref(h)) == -1
+    assert b'aw2ali entry 5' in L.nipet_last_error()
+    assert L.nipet_norm_sino(None, None, None, None, None) == -1
+    assert L.nipet_varon(None, None, None, 0, 10, None) == -1
+    assert L.nipet_rnd_estimate(None, None, 10, None, None, None) == -1
+    assert L.nipet_psf_conv(None, None, None, None, 1, None) == -1
