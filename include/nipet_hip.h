@@ -70,7 +70,8 @@ int nipet_geom_info(const NipetGeom *g, int *naw, int64_t *nseg_total, int *nli,
 /* Copy the transaxial intersection lists to host arrays: offsets[naw+1],
  * col[nseg_total] (voxel column u + 320 v), t[nseg_total] (segment start, cm),
  * len[nseg_total] (segment length, cm). The equivalent of the reference's
- * tv/tt tables (petprj.tprj, prj/src/prj_module.cu:70-224). Synchronous. */
+ * tv/tt tables (petprj.tprj, prj/src/prj_module.cu:70-224). col, t and len
+ * may all be NULL: then only the offsets are copied. Synchronous. */
 int nipet_tx_lists(const NipetGeom *g, int64_t *offsets, int32_t *col, float *t, float *len);
 
 /* Forward projection (petprj.fprj, prj/src/prj_module.cu:232-425;

@@ -514,15 +514,17 @@ int nipet_geom_info(const NipetGeom *g, int *naw, int64_t *nseg_total, int *nli,
 }
 
 int nipet_tx_lists(const NipetGeom *g, int64_t *offsets, int32_t *col, float *t, float *len) {
-  if (!g || !offsets || !col || !t || !len) {
-    set_error("nipet_tx_lists: NULL argument");
+  const bool lists = col || t || len;
+  if (!g || !offsets || (lists && (!col || !t || !len))) {
+    set_error("nipet_tx_lists: NULL argument (offsets required; col, t, len all or none)");
     return NIPET_EINVAL;
   }
   std::vector<int32_t> off(g->naw + 1);
-  std::vector<SegRec> sg(g->nseg);
   NIPET_HIP(hipMemcpy(off.data(), g->off, (g->naw + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
-  NIPET_HIP(hipMemcpy(sg.data(), g->seg, g->nseg * sizeof(SegRec), hipMemcpyDeviceToHost));
   for (int i = 0; i <= g->naw; i++) offsets[i] = off[i];
+  if (!lists) return NIPET_OK;
+  std::vector<SegRec> sg(g->nseg);
+  NIPET_HIP(hipMemcpy(sg.data(), g->seg, g->nseg * sizeof(SegRec), hipMemcpyDeviceToHost));
   for (int64_t k = 0; k < g->nseg; k++) {
     col[k] = sg[k].colz / NZ;
     t[k] = (float)sg[k].s;

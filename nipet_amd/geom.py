@@ -61,6 +61,15 @@ class DeviceGeom:
             _lib._lib.nipet_geom_destroy(h)
             self.handle = None
 
+    def seg_counts(self):
+        """Segments per transaxial bin (int64 [naw]), cached host copy."""
+        if getattr(self, '_segc', None) is None:
+            off = np.zeros(self.naw + 1, np.int64)
+            _lib.check(_lib.lib().nipet_tx_lists(self.handle, off.ctypes.data, None, None, None),
+                       'nipet_tx_lists')
+            self._segc = np.diff(off)
+        return self._segc
+
     def tx_lists(self):
         """(offsets int64 [naw+1], col int32 (u + 320 v), t float32, len float32)."""
         off = np.zeros(self.naw + 1, np.int64)

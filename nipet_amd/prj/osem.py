@@ -25,6 +25,20 @@ def _torch():
     return torch
 
 
+def _longest_first(geom, subs):
+    """A subset's bins ordered by decreasing segment count (device int32).
+    Workgroups are dispatched in bin order, so the long rays start first and
+    the last wave of workgroups holds short ones (a shorter tail). FP and BP
+    walk the same list, so the OSEM result does not depend on the order."""
+    torch = _torch()
+    rows = subs.cpu().numpy() if isinstance(subs, torch.Tensor) else np.asarray(subs)
+    segc = geom.seg_counts()
+    valid = (rows >= 0) & (rows < len(segc))
+    key = np.where(valid, segc[np.clip(rows, 0, len(segc) - 1)], -1)
+    order = np.argsort(-key, kind='stable')
+    return _dev(np.ascontiguousarray(rows[order], np.int32), torch.int32)
+
+
 def _dev(x, dtype):
     torch = _torch()
     if isinstance(x, torch.Tensor):
@@ -122,8 +136,7 @@ def osem_prepare(scanner_params, psng, asng, nsng, rsng=None, ssng=None, mask_ra
                                           _lib.stream_handle()), 'nipet_osem_corr')
     if subsets is None:
         subsets = [get_subsets14(k, scanner_params)[0] for k in range(14)]
-    subs = [x.to('cuda', torch.int32).contiguous() if isinstance(x, torch.Tensor) else
-            _dev(np.asarray(x, np.int32), torch.int32) for x in subsets]
+    subs = [_longest_first(geom, x) for x in subsets]
     krnl = _psf_kernel(psf)
     sens = torch.stack([sensitivity(geom, an, x) for x in subs])
     if krnl is not None:
@@ -248,8 +261,7 @@ def osem_prepare_batch(scanner_params, prompts, asng, nsng, rsng=None, ssng=None
     del tmp, one
     if subsets is None:
         subsets = [get_subsets14(k, scanner_params)[0] for k in range(14)]
-    subs = [x.to('cuda', torch.int32).contiguous() if isinstance(x, torch.Tensor) else
-            _dev(np.asarray(x, np.int32), torch.int32) for x in subsets]
+    subs = [_longest_first(geom, x) for x in subsets]
     krnl = _psf_kernel(psf)
     if sens is None:
         sens = torch.stack([sensitivity(geom, an, x) for x in subs])
