@@ -57,7 +57,8 @@ class DeviceGeom:
 
     def __del__(self):
         h = getattr(self, 'handle', None)
-        if h is not None and h.value and _lib._lib is not None:
+        # at interpreter exit the module globals may already be gone
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
             _lib._lib.nipet_geom_destroy(h)
             self.handle = None
 

@@ -109,7 +109,8 @@ class LmHistogram:
 
     def __del__(self):
         h = getattr(self, 'handle', None)
-        if h is not None and h.value and _lib._lib is not None:
+        # at interpreter exit the module globals may already be gone
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
             _lib._lib.nipet_lm_destroy(h)
             self.handle = None
 
@@ -307,7 +308,8 @@ class Randoms:
 
     def __del__(self):
         h = getattr(self, 'handle', None)
-        if h is not None and h.value and _lib._lib is not None:
+        # at interpreter exit the module globals may already be gone
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
             _lib._lib.nipet_rnd_destroy(h)
             self.handle = None
 

@@ -119,7 +119,8 @@ class NormSino:
 
     def __del__(self):
         h = getattr(self, 'handle', None)
-        if h is not None and h.value and _lib._lib is not None:
+        # at interpreter exit the module globals may already be gone
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
             _lib._lib.nipet_norm_destroy(h)
             self.handle = None
 
