@@ -423,7 +423,9 @@ def main():
                 'I_per_subset': float(terms.mean()),
                 'fp_ms': round(t_fp * 1e3, 4), 'bp_ms': round(t_bp * 1e3, 4),
                 'fp_GBs': round(b_fp / t_fp / 1e9, 1), 'bp_GBs': round(b_bp / t_bp / 1e9, 1),
-                'update_ms': round(kt.mean_ms(_lib.PROF_OSEM_UPDATE), 4)}
+                'update_ms': round(kt.mean_ms(_lib.PROF_OSEM_UPDATE), 4),
+                # first iteration: sensitivity and ratio back-projected together
+                'bp_pair_ms': round(kt.mean_ms(_lib.PROF_BPRJ_PAIR), 4)}
     out = {
         'metric': METRIC, 'value': round(value, 3), 'unit': 'pairs/s', 'n_gpus': 1,
         'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': round(ms_step, 3),

@@ -138,6 +138,19 @@ int nipet_osem_subset(const NipetGeom *g, float *img, const uint16_t *prompts, c
                       const float *sens, const uint8_t *mask, const int32_t *subs, int nprj,
                       float *work_sino, float *work_img, void *stream);
 
+/* nipet_osem_subset for a subset whose sensitivity image is not yet known
+ * (the first OSEM iteration): sens = BP_sub(an[subs]) (the subset's image of
+ * prj/mmrrec.py:291-298, which the reference computes before the loop) and
+ * b = BP_sub(e) are back-projected together, as one two-image batch through
+ * one traversal of the subset's rays; sens is written out and the update is
+ * that of nipet_osem_subset. an: float [naw][nsn]; sens: float image (output);
+ * work_sino: float [nprj][nsn]; work_sino2: float [nprj][nsn][2]; work_img2:
+ * float [voxel][2]. Span 11 only (nsn <= 837). */
+int nipet_osem_subset_sens(const NipetGeom *g, float *img, const uint16_t *prompts,
+                           const float *rho, const float *an, float *sens, const uint8_t *mask,
+                           const int32_t *subs, int nprj, float *work_sino, float *work_sino2,
+                           float *work_img2, void *stream);
+
 /* Batched OSEM subset update of nb images [voxel][nb]: prompts [naw][nsn][nb];
  * rho shared [naw][nsn] (rho_per_rep = 0) or [naw][nsn][nb]; sens shared
  * [voxel] (sens_per_rep = 0) or [voxel][nb]; mask shared; work_sino
@@ -300,6 +313,8 @@ int nipet_philox4x32(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
 #define NIPET_PROF_OSEM_UPDATE 2
 #define NIPET_PROF_FILL 3
 #define NIPET_PROF_HIST 4
+#define NIPET_PROF_OTHER 5      /* small helper kernels */
+#define NIPET_PROF_BPRJ_PAIR 6  /* two-image BP of nipet_osem_subset_sens */
 int nipet_prof_enable(int on);
 /* Largest replicate sub-batch (4, 2 or 1) that one FP (kernel =
  * NIPET_PROF_FPRJ) or BP (NIPET_PROF_BPRJ) launch covers in the batched entry

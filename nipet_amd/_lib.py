@@ -15,7 +15,7 @@ LIBPATH = os.path.join(_HERE, 'libnipet_hip.so')
 _lock = threading.Lock()
 _lib = None
 
-PROF_FPRJ, PROF_BPRJ, PROF_OSEM_UPDATE, PROF_FILL, PROF_HIST = range(5)
+PROF_FPRJ, PROF_BPRJ, PROF_OSEM_UPDATE, PROF_FILL, PROF_HIST, PROF_OTHER, PROF_BPRJ_PAIR = range(7)
 
 
 class NipetError(RuntimeError):
@@ -85,6 +85,7 @@ def lib():
             'nipet_osem_subset_rm_b': ([P, P, P, P, I, P, I, P, P, I, I, P, P, P, P, P], I),
             'nipet_tune_get': ([I, P], I),
             'nipet_osem_subset': ([P, P, P, P, P, P, P, I, P, P, P], I),
+            'nipet_osem_subset_sens': ([P, P, P, P, P, P, P, P, I, P, P, P, P], I),
             'nipet_fprj_b': ([P, P, P, P, I, I, I, P], I),
             'nipet_bprj_b': ([P, P, P, P, I, I, P], I),
             'nipet_osem_subset_b': ([P, P, P, P, I, P, I, P, P, I, I, P, P, P], I),
@@ -149,7 +150,7 @@ class KernelTimer:
         L = lib()
         check(L.nipet_prof_enable(0), 'nipet_prof_enable')
         self.times = {}
-        for k in range(5):
+        for k in range(PROF_BPRJ_PAIR + 1):
             ms, n = ct.c_double(), ct.c_int64()
             check(L.nipet_prof_read(k, ct.byref(ms), ct.byref(n)), 'nipet_prof_read')
             self.times[k] = (ms.value, n.value)

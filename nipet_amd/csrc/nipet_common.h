@@ -67,7 +67,9 @@ int psf_conv(float *out, const float *in, float *tmp, const float *krnl, int nb,
 
 // Opt-in kernel timing (nipet_prof_*): HIP events recorded around launches on
 // the launch stream, resolved when read.
-enum ProfKernel { PK_FPRJ = 0, PK_BPRJ, PK_OSEM_UPDATE, PK_FILL, PK_HIST, PK_OTHER, PK_COUNT };
+enum ProfKernel {
+  PK_FPRJ = 0, PK_BPRJ, PK_OSEM_UPDATE, PK_FILL, PK_HIST, PK_OTHER, PK_BPRJ_PAIR, PK_COUNT
+};
 void prof_begin(int kernel, hipStream_t st);
 void prof_end(int kernel, hipStream_t st);
 

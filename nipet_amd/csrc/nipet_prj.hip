@@ -968,6 +968,36 @@ __global__ void k_osem_update(float *img, const float *b, const float *sens, con
   }
 }
 
+// The first-iteration pair: e (the subset's ratio sinogram, [nprj][nsn]) and
+// the a n rows of the subset's bins, interleaved [nprj][nsn][2] for one
+// two-image back projection (zero rows for bins outside 0..naw-1).
+__global__ void k_pack_pair(float2 *out, const float *e, const float *an, const int32_t *subs,
+                            int nsn, int naw, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    const int64_t row = i / nsn;
+    const int aw = subs[row];
+    const float a = (aw >= 0 && aw < naw) ? an[(int64_t)aw * nsn + (i - row * nsn)] : 0.f;
+    out[i] = make_float2(e[i], a);
+  }
+}
+
+// k_osem_update (nb = 1) from the two-image back projection b2[v] = (BP(e),
+// BP(a n)): the second image is the subset's sensitivity, stored to sens.
+__global__ void k_osem_update_pair(float *img, float *sens, const float2 *b2,
+                                   const uint8_t *mask, int64_t nvox) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < nvox; i += stride) {
+    const float2 bs = b2[i];
+    const float sv = bs.y;
+    sens[i] = sv;
+    float q = (sv > -EPS && sv < EPS) ? 0.f : bs.x / sv;
+    img[i] = mask[i] ? img[i] * q : 0.f;
+  }
+}
+
 // Siemens histogram with replicates innermost, in[(s * nbinang + ali) * nrep + r]
 // (uint32 counts) -> device prompts out[(aw * nsn + s) * nb + b] (uint16,
 // saturating at 65535) for replicates r = r0 + b, b < nb.
@@ -1349,6 +1379,47 @@ int nipet_osem_subset(const NipetGeom *g, float *img, const uint16_t *prompts, c
                       float *work_sino, float *work_img, void *stream) {
   return nipet_osem_subset_b(g, img, prompts, rho, 0, sens, 0, mask, subs, nprj, 1, work_sino,
                              work_img, stream);
+}
+
+int nipet_osem_subset_sens(const NipetGeom *g, float *img, const uint16_t *prompts,
+                           const float *rho, const float *an, float *sens, const uint8_t *mask,
+                           const int32_t *subs, int nprj, float *work_sino, float *work_sino2,
+                           float *work_img2, void *stream) {
+  int rc = check_geom(g, "nipet_osem_subset_sens");
+  if (rc) return rc;
+  if (!subs || nprj <= 0 || nprj > g->naw) {
+    set_error("nipet_osem_subset_sens: subs must be given with 0 < nprj <= naw");
+    return NIPET_EINVAL;
+  }
+  if ((rc = check_nprj(g, subs, nprj, 2, "nipet_osem_subset_sens"))) return rc;
+  if (!img || !prompts || !rho || !an || !sens || !mask || !work_sino || !work_sino2 ||
+      !work_img2 || sens == img || work_sino == work_sino2) {
+    set_error("nipet_osem_subset_sens: NULL or aliased buffer");
+    return NIPET_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t nel = (int64_t)nprj * g->nsn;
+  prof_begin(PK_FPRJ, st);
+  launch_fprj<2>(g, work_sino, img, subs, nprj, prompts, rho, 0, 1, st);
+  prof_end(PK_FPRJ, st);
+  prof_begin(PK_OTHER, st);
+  k_pack_pair<<<ew_blocks(nel), 256, 0, st>>>((float2 *)work_sino2, work_sino, an, subs, g->nsn,
+                                               g->naw, nel);
+  prof_end(PK_OTHER, st);
+  prof_begin(PK_FILL, st);
+  k_fill<<<ew_blocks(NVOX * 2), 256, 0, st>>>(work_img2, 0.f, NVOX * 2);
+  prof_end(PK_FILL, st);
+  // both images through one traversal of the rays (independent of the
+  // batch autotuner's width)
+  prof_begin(PK_BPRJ_PAIR, st);
+  k_bprj<2><<<nprj, BP_THREADS, 0, st>>>(view(g), work_img2, work_sino2, subs, Batch{2, 0});
+  prof_end(PK_BPRJ_PAIR, st);
+  prof_begin(PK_OSEM_UPDATE, st);
+  k_osem_update_pair<<<ew_blocks(NVOX), 256, 0, st>>>(img, sens, (const float2 *)work_img2, mask,
+                                                      NVOX);
+  prof_end(PK_OSEM_UPDATE, st);
+  NIPET_HIP(hipGetLastError());
+  return NIPET_OK;
 }
 
 }  // extern "C"

@@ -7,7 +7,12 @@ Differences in structure (not in arithmetic) from the reference:
   iteration, ``prj/src/recon.cu:333-419``);
 * the transaxial intersection lists are built once per geometry;
 * per subset, three launches: FP fused with ``y / (FP + rho)``, the gather BP,
-  and the fused ``img * BP / sens`` masked update (``nipet_osem_subset``).
+  and the fused ``img * BP / sens`` masked update (``nipet_osem_subset``);
+* without resolution modelling (span 11), a subset's sensitivity image is
+  back-projected in its first update, in the same launch as the ratio
+  (``nipet_osem_subset_sens``: two images per traversal of the rays), not in a
+  separate pass before the loop (``prj/mmrrec.py:291-298``). The images are
+  the same; ``OsemState.sensitivity()`` completes the missing ones on demand.
 """
 import numpy as np
 
@@ -49,24 +54,48 @@ def _dev(x, dtype):
 class OsemState:
     """Everything one OSEM reconstruction keeps on the device."""
 
-    def __init__(self, geom, subsets, prompts, rho, an, sens, mask, krnl=None):
+    def __init__(self, geom, subsets, prompts, rho, an, sens, mask, krnl=None, pending=None):
         torch = _torch()
         self.geom = geom
         self.subsets = subsets
         self.nprj = max(len(s) for s in subsets)
-        self.prompts, self.rho, self.an, self.sens, self.mask = prompts, rho, an, sens, mask
+        self.prompts, self.rho, self.an, self._sens, self.mask = prompts, rho, an, sens, mask
+        # subsets whose sensitivity image is still to be back-projected (in
+        # their first update)
+        self.pending = set() if pending is None else set(pending)
         self.work_sino = torch.empty((self.nprj, geom.nsn), dtype=torch.float32, device='cuda')
         self.work_img = torch.empty((320, 320, 127), dtype=torch.float32, device='cuda')
         # resolution modelling: host kernel (3, 17) and a second work image
         self.krnl = krnl
         self.work_img2 = None if krnl is None else torch.empty_like(self.work_img)
+        self._pair = None
+
+    @property
+    def sens(self):
+        """Sensitivity images [nsub, 320, 320, 127], completing any not yet
+        back-projected."""
+        for k in sorted(self.pending):
+            sensitivity(self.geom, self.an, self.subsets[k], out=self._sens[k])
+        self.pending.clear()
+        return self._sens
+
+    def pair_work(self):
+        """Work buffers of the two-image first update (allocated while subsets
+        are pending)."""
+        torch = _torch()
+        if self._pair is None:
+            self._pair = (torch.empty((self.nprj, self.geom.nsn, 2), dtype=torch.float32,
+                                      device='cuda'),
+                          torch.empty((320, 320, 127, 2), dtype=torch.float32, device='cuda'))
+        return self._pair
 
 
-def sensitivity(geom, an, subs):
+def sensitivity(geom, an, subs, out=None):
     """BP_subset(a*n) (``prj/mmrrec.py:291-298``) for one subset (device tensors)."""
     torch = _torch()
     rows = an.index_select(0, subs.long())
-    out = torch.empty((320, 320, 127), dtype=torch.float32, device='cuda')
+    if out is None:
+        out = torch.empty((320, 320, 127), dtype=torch.float32, device='cuda')
     _lib.check(_lib.lib().nipet_bprj(geom.handle, _lib.ptr(out), _lib.ptr(rows), _lib.ptr(subs),
                                      len(subs), _lib.stream_handle()), 'nipet_bprj')
     return out
@@ -138,7 +167,13 @@ def osem_prepare(scanner_params, psng, asng, nsng, rsng=None, ssng=None, mask_ra
         subsets = [get_subsets14(k, scanner_params)[0] for k in range(14)]
     subs = [_longest_first(geom, x) for x in subsets]
     krnl = _psf_kernel(psf)
-    sens = torch.stack([sensitivity(geom, an, x) for x in subs])
+    if krnl is None and geom.nsn <= 837:
+        # back-projected with each subset's first update
+        sens = torch.empty((len(subs), 320, 320, 127), dtype=torch.float32, device='cuda')
+        pending = range(len(subs))
+    else:
+        sens = torch.stack([sensitivity(geom, an, x) for x in subs])
+        pending = None
     if krnl is not None:
         tmp = torch.empty((320, 320, 127), dtype=torch.float32, device='cuda')
         for k in range(len(subs)):
@@ -148,7 +183,7 @@ def osem_prepare(scanner_params, psng, asng, nsng, rsng=None, ssng=None, mask_ra
         mask = get_cylinder(Cnt, rad=mask_radius, xo=0, yo=0, unival=1, gpu_dim=True) > 0.9
     msk = _dev(mask if isinstance(mask, torch.Tensor) else np.asarray(mask, np.uint8),
                torch.uint8)
-    return OsemState(geom, subs, prompts, rho, an, sens, msk, krnl)
+    return OsemState(geom, subs, prompts, rho, an, sens, msk, krnl, pending)
 
 
 def osem_iteration(state, img):
@@ -163,8 +198,19 @@ def osem_iteration(state, img):
                 state.krnl.ctypes.data, _lib.ptr(state.work_sino), _lib.ptr(state.work_img),
                 _lib.ptr(state.work_img2), st), 'nipet_osem_subset_rm_b')
             continue
+        if k in state.pending:
+            ws2, wi2 = state.pair_work()
+            _lib.check(L.nipet_osem_subset_sens(
+                state.geom.handle, _lib.ptr(img), _lib.ptr(state.prompts), _lib.ptr(state.rho),
+                _lib.ptr(state.an), _lib.ptr(state._sens[k]), _lib.ptr(state.mask),
+                _lib.ptr(subs), len(subs), _lib.ptr(state.work_sino), _lib.ptr(ws2),
+                _lib.ptr(wi2), st), 'nipet_osem_subset_sens')
+            state.pending.discard(k)
+            if not state.pending:
+                state._pair = None
+            continue
         _lib.check(L.nipet_osem_subset(state.geom.handle, _lib.ptr(img), _lib.ptr(state.prompts),
-                                       _lib.ptr(state.rho), _lib.ptr(state.sens[k]),
+                                       _lib.ptr(state.rho), _lib.ptr(state._sens[k]),
                                        _lib.ptr(state.mask), _lib.ptr(subs), len(subs),
                                        _lib.ptr(state.work_sino), _lib.ptr(state.work_img), st),
                    'nipet_osem_subset')
