@@ -273,17 +273,10 @@ int nipet_prof_read(int kernel, double *total_ms, int64_t *count) {
 }
 
 static void geom_free(Geom *g) {
-  hipFree(g->off);
-  hipFree(g->seg);
-  hipFree(g->atn);
-  hipFree(g->lmax);
-  hipFree(g->li_z);
-  hipFree(g->li_sn);
-  hipFree(g->li_nosi);
-  hipFree(g->dr_li);
-  hipFree(g->sn_off);
-  hipFree(g->sn_mem);
-  hipFree(g->fp_slot);
+  // teardown: a failing free leaves nothing to undo
+  void *bufs[] = {g->off,   g->seg,    g->atn,    g->lmax,   g->li_z,   g->li_sn,
+                  g->li_nosi, g->dr_li, g->sn_off, g->sn_mem, g->fp_slot};
+  for (void *p : bufs) (void)hipFree(p);
 }
 
 static int geom_build(const NipetGeomDesc *d, NipetGeom *g) {
@@ -408,6 +401,13 @@ static int geom_build(const NipetGeomDesc *d, NipetGeom *g) {
   short2 *d_s2c = nullptr;
   int32_t *d_cnt = nullptr;
   int *d_err = nullptr;
+  // the build's temporaries are freed on every return path
+  struct Tmp {
+    void **p[4];
+    ~Tmp() {
+      for (void **q : p) (void)hipFree(*q);
+    }
+  } tmp{{(void **)&d_crs, (void **)&d_s2c, (void **)&d_cnt, (void **)&d_err}};
   NIPET_HIP(hipMalloc(&d_crs, d->ncrs * 4 * sizeof(float)));
   NIPET_HIP(hipMalloc(&d_s2c, d->naw * sizeof(short2)));
   NIPET_HIP(hipMalloc(&d_cnt, d->naw * sizeof(int32_t)));
@@ -462,10 +462,6 @@ static int geom_build(const NipetGeomDesc *d, NipetGeom *g) {
         return NIPET_EINVAL;
       }
   }
-  hipFree(d_crs);
-  hipFree(d_s2c);
-  hipFree(d_cnt);
-  hipFree(d_err);
   return NIPET_OK;
 }
 
