@@ -11,7 +11,7 @@ import math
 
 import numpy as np
 
-__all__ = ['convert2e7', 'convert2dev', 'get_cylinder']
+__all__ = ['convert2e7', 'convert2dev', 'get_cylinder', 'cylinder_disc']
 
 
 def convert2e7(img, Cnt):
@@ -37,16 +37,14 @@ def convert2dev(im, Cnt):
     return np.ascontiguousarray(np.transpose(sub, (1, 2, 0)), dtype=np.float32)
 
 
-def get_cylinder(Cnt, rad=25, xo=0, yo=0, unival=1, gpu_dim=False):
-    """Uniform cylinder of value ``unival``, transaxial radius ``rad`` [cm],
-    centre (xo, yo) [cm], in the Siemens shape (or device shape if
-    ``gpu_dim``).
+def cylinder_disc(Cnt, rad=25, xo=0, yo=0, unival=1):
+    """Transaxial (344, 344) slice of ``get_cylinder``.
 
     The disc is rasterised by sweeping half-angles t in [0, pi) in steps of
     pi/720 and filling, for each t, the column u(x=xo+rad cos t) over the rows
     covered by y in [-y+2yo, y) sampled at half a voxel, with
     u = 172 + floor(x/vox), v = 172 - ceil(y/vox) (the same sampling as the
-    reference, so the mask voxels agree)."""
+    reference, ``img/mmrimg.py:208-227``, so the mask voxels agree)."""
     vox = Cnt['SO_VXY']
     nx, ny = Cnt['SO_IMX'], Cnt['SO_IMY']
     disc = np.zeros((nx, ny), dtype=np.float32)
@@ -57,6 +55,14 @@ def get_cylinder(Cnt, rad=25, xo=0, yo=0, unival=1, gpu_dim=False):
         v = np.int32(.5 * nx - np.ceil(yf / vox))
         u = np.int32(.5 * ny + np.floor(x / vox))
         disc[v, u] = unival
+    return disc
+
+
+def get_cylinder(Cnt, rad=25, xo=0, yo=0, unival=1, gpu_dim=False):
+    """Uniform cylinder of value ``unival``, transaxial radius ``rad`` [cm],
+    centre (xo, yo) [cm], in the Siemens shape (or device shape if
+    ``gpu_dim``): ``cylinder_disc`` repeated over the planes."""
+    disc = cylinder_disc(Cnt, rad, xo, yo, unival)
     nvz = Cnt.get('rSO_IMZ', Cnt['SO_IMZ'])
     img = np.repeat(disc[None], nvz, axis=0)
     if gpu_dim:

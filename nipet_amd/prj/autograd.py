@@ -225,9 +225,8 @@ def map_recon(scanner_params, prompts, norm, rs, mu, subsets, beta=1e-3, steps=2
         if sens is None:
             sens = sum(p.bp(a) for p, a in zip(prjs, an))
         if mask is None:
-            from ..img.mmrimg import get_cylinder
-            mask = torch.from_numpy(get_cylinder(scanner_params['Cnt'], rad=29., xo=0, yo=0,
-                                                 unival=1, gpu_dim=True) > 0.9).cuda()
+            from .osem import cylinder_mask
+            mask = cylinder_mask(scanner_params['Cnt'], 29.).bool()
         x = torch.ones((320, 320, 127), device='cuda') if img0 is None else img0.clone()
         x = torch.where(mask, x, torch.zeros_like(x))
         pre_scale = torch.where(sens > 0, 1.0 / sens, torch.zeros_like(sens))

@@ -19,7 +19,7 @@ import numpy as np
 from .. import _lib
 from ..geom import get_geom
 
-__all__ = ['OsemState', 'osem_prepare', 'osem_run', 'sensitivity']
+__all__ = ['OsemState', 'osem_prepare', 'osem_run', 'sensitivity', 'cylinder_mask']
 
 
 NVOX = 320 * 320 * 127
@@ -42,6 +42,30 @@ def _longest_first(geom, subs):
     key = np.where(valid, segc[np.clip(rows, 0, len(segc) - 1)], -1)
     order = np.argsort(-key, kind='stable')
     return _dev(np.ascontiguousarray(rows[order], np.int32), torch.int32)
+
+
+_MASKS = {}
+
+
+def cylinder_mask(Cnt, radius=29.):
+    """Device uint8 mask (320, 320, nz) equal to ``get_cylinder(Cnt, rad=radius,
+    gpu_dim=True) > 0.9`` (the default OSEM mask, ``prj/mmrrec.py:303``),
+    made from the one transaxial disc broadcast over the planes on the device and
+    cached per device. Callers must not modify it."""
+    torch = _torch()
+    from ..img.mmrimg import cylinder_disc
+    nvz = Cnt.get('rSO_IMZ', Cnt['SO_IMZ'])
+    key = (float(radius), Cnt['SO_IMX'], Cnt['SZ_IMX'], float(Cnt['SO_VXY']), nvz,
+           torch.cuda.current_device())
+    m = _MASKS.get(key)
+    if m is None:
+        margin = (Cnt['SO_IMX'] - Cnt['SZ_IMX']) // 2
+        disc = cylinder_disc(Cnt, rad=radius)[margin:margin + Cnt['SZ_IMY'],
+                                              margin:margin + Cnt['SZ_IMX']] > 0.9
+        d = torch.from_numpy(np.ascontiguousarray(disc, np.uint8)).to('cuda')
+        m = d[:, :, None].expand(d.shape[0], d.shape[1], nvz).contiguous()
+        _MASKS[key] = m
+    return m
 
 
 def _dev(x, dtype):
@@ -136,7 +160,6 @@ def osem_prepare(scanner_params, psng, asng, nsng, rsng=None, ssng=None, mask_ra
     ``psf``: (3, 17) kernel from ``psf_config`` (resolution modelling: the
     sensitivity images are convolved once, ``prj/src/recon.cu:441-445``)."""
     torch = _torch()
-    from ..img.mmrimg import get_cylinder
     from .mmrrec import get_subsets14
     geom = get_geom(scanner_params)
     Cnt = scanner_params['Cnt']
@@ -180,7 +203,7 @@ def osem_prepare(scanner_params, psng, asng, nsng, rsng=None, ssng=None, mask_ra
             tmp.copy_(sens[k])
             psf_conv(tmp, krnl, out=sens[k], tmp=tmp)
     if mask is None:
-        mask = get_cylinder(Cnt, rad=mask_radius, xo=0, yo=0, unival=1, gpu_dim=True) > 0.9
+        mask = cylinder_mask(Cnt, mask_radius)
     msk = _dev(mask if isinstance(mask, torch.Tensor) else np.asarray(mask, np.uint8),
                torch.uint8)
     return OsemState(geom, subs, prompts, rho, an, sens, msk, krnl, pending)
@@ -269,7 +292,6 @@ def osem_prepare_batch(scanner_params, prompts, asng, nsng, rsng=None, ssng=None
     per replicate (Naw, nsn, nb). ``sens`` may be passed from an earlier batch
     with the same a n (a (14, 320, 320, 127) tensor)."""
     torch = _torch()
-    from ..img.mmrimg import get_cylinder
     from .mmrrec import get_subsets14
     geom = get_geom(scanner_params)
     Cnt = scanner_params['Cnt']
@@ -317,7 +339,7 @@ def osem_prepare_batch(scanner_params, prompts, asng, nsng, rsng=None, ssng=None
                 tmp.copy_(sens[k])
                 psf_conv(tmp, krnl, out=sens[k], tmp=tmp)
     if mask is None:
-        mask = get_cylinder(Cnt, rad=mask_radius, xo=0, yo=0, unival=1, gpu_dim=True) > 0.9
+        mask = cylinder_mask(Cnt, mask_radius)
     msk = _dev(mask if isinstance(mask, torch.Tensor) else np.asarray(mask, np.uint8),
                torch.uint8)
     return OsemBatch(geom, subs, p, rho, per_rep, sens, msk, nb, work=work, krnl=krnl)

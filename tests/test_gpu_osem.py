@@ -105,6 +105,20 @@ def test_first_update_pair_matches_separate_sensitivity(torch, params, subsets):
     assert bool((x_lazy[~torch.from_numpy(mask).cuda()] == 0).all())
 
 
+def test_cylinder_mask_equals_reference_mask(torch, params):
+    """The device mask built from one transaxial disc equals the reference's
+    get_cylinder(..., gpu_dim=True) > 0.9 (prj/mmrrec.py:303), voxel for voxel."""
+    import nipet_amd as nip
+    from nipet_amd.prj.osem import cylinder_mask
+    Cnt = params['Cnt']
+    for rad in (29., 12.5):
+        ref = nip.get_cylinder(Cnt, rad=rad, xo=0, yo=0, unival=1, gpu_dim=True) > 0.9
+        m = cylinder_mask(Cnt, rad)
+        assert m.dtype == torch.uint8 and tuple(m.shape) == ref.shape
+        assert np.array_equal(m.cpu().numpy().astype(bool), ref)
+        assert cylinder_mask(Cnt, rad) is m
+
+
 def test_mlem_count_preservation(torch, params, subsets):
     import nipet_amd as nip
     from nipet_amd.prj.osem import osem_prepare, osem_run
