@@ -133,7 +133,8 @@ int nipet_osem_corr(float *rho, float *an, const float *a, const float *n, const
  *   e = FP_sub(img) + rho[subs]; e = y[subs] / e (0 where |e| < 1e-6);
  *   b = BP_sub(e); img = mask ? img * (b / sens) : 0 (b/sens = 0 where |sens| < 1e-6).
  * prompts: uint16 [naw][nsn]; rho: float [naw][nsn]; sens: float image of this
- * subset; mask: uint8 image; work_sino: float [nprj][nsn]; work_img: float image. */
+ * subset; mask: uint8 image; work_sino: float [nprj][nsn]; work_img: float image
+ * distinct from img and sens (aliased work buffers are rejected). */
 int nipet_osem_subset(const NipetGeom *g, float *img, const uint16_t *prompts, const float *rho,
                       const float *sens, const uint8_t *mask, const int32_t *subs, int nprj,
                       float *work_sino, float *work_img, void *stream);
@@ -145,7 +146,8 @@ int nipet_osem_subset(const NipetGeom *g, float *img, const uint16_t *prompts, c
  * one traversal of the subset's rays; sens is written out and the update is
  * that of nipet_osem_subset. an: float [naw][nsn]; sens: float image (output);
  * work_sino: float [nprj][nsn]; work_sino2: float [nprj][nsn][2]; work_img2:
- * float [voxel][2]. Span 11 only (nsn <= 837). */
+ * float [voxel][2] (work buffers distinct from img and sens). Span 11 only
+ * (nsn <= 837). */
 int nipet_osem_subset_sens(const NipetGeom *g, float *img, const uint16_t *prompts,
                            const float *rho, const float *an, float *sens, const uint8_t *mask,
                            const int32_t *subs, int nprj, float *work_sino, float *work_sino2,

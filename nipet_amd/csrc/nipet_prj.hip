@@ -1313,8 +1313,9 @@ int nipet_osem_subset_b(const NipetGeom *g, float *img, const uint16_t *prompts,
     return NIPET_EINVAL;
   }
   if ((rc = check_nprj(g, subs, nprj, nb, "nipet_osem_subset"))) return rc;
-  if (!img || !prompts || !rho || !sens || !mask || !work_sino || !work_img) {
-    set_error("nipet_osem_subset: NULL buffer");
+  if (!img || !prompts || !rho || !sens || !mask || !work_sino || !work_img || work_img == img ||
+      work_img == sens) {
+    set_error("nipet_osem_subset: NULL or aliased buffer");
     return NIPET_EINVAL;
   }
   hipStream_t st = (hipStream_t)stream;
@@ -1348,7 +1349,8 @@ int nipet_osem_subset_rm_b(const NipetGeom *g, float *img, const uint16_t *promp
   }
   if ((rc = check_nprj(g, subs, nprj, nb, "nipet_osem_subset_rm"))) return rc;
   if (!img || !prompts || !rho || !sens || !mask || !krnl || !work_sino || !work_img ||
-      !work_img2 || work_img == work_img2) {
+      !work_img2 || work_img == work_img2 || work_img == img || work_img2 == img ||
+      work_img == sens || work_img2 == sens) {
     set_error("nipet_osem_subset_rm: NULL or aliased buffer");
     return NIPET_EINVAL;
   }
@@ -1393,7 +1395,8 @@ int nipet_osem_subset_sens(const NipetGeom *g, float *img, const uint16_t *promp
   }
   if ((rc = check_nprj(g, subs, nprj, 2, "nipet_osem_subset_sens"))) return rc;
   if (!img || !prompts || !rho || !an || !sens || !mask || !work_sino || !work_sino2 ||
-      !work_img2 || sens == img || work_sino == work_sino2) {
+      !work_img2 || sens == img || work_sino == work_sino2 || work_img2 == img ||
+      work_img2 == sens) {
     set_error("nipet_osem_subset_sens: NULL or aliased buffer");
     return NIPET_EINVAL;
   }

@@ -147,6 +147,54 @@ def test_operator_shapes(torch, params, img):
         nip.frwd_prj(np.zeros((10, 10, 10), np.float32), params)
 
 
+def test_ragged_and_invalid_rows_through_c_abi(torch, params, img):
+    """Edge cases through the C-ABI: the shortest and the longest transaxial
+    bin projected alone (nprj = 1) against the oracle; bin indices outside
+    0..naw-1 in a subset list give zero FP rows and add nothing to the BP;
+    argument errors come back as status codes with a message."""
+    from oracle import oracle as O
+
+    from nipet_amd import _lib
+    from nipet_amd.geom import get_geom
+    g = get_geom(params)
+    naw = params['txLUT']['Naw']
+    segc = g.seg_counts()
+    L, st = _lib.lib(), _lib.stream_handle()
+    x = torch.from_numpy(img).cuda()
+    for aw in (int(np.argmin(segc)), int(np.argmax(segc))):
+        print(f'single bin {aw} ({segc[aw]} segments)', flush=True)
+        sub = torch.tensor([aw], dtype=torch.int32, device='cuda')
+        out = torch.full((1, 837), 7.0, device='cuda')
+        _lib.check(L.nipet_fprj(g.handle, out.data_ptr(), x.data_ptr(), sub.data_ptr(), 1, 0, st))
+        ref = O.fprj(params, img, np.array([aw], np.int32))
+        if np.abs(ref).max() == 0:
+            assert float(out.abs().max()) == 0.0
+        else:
+            assert _rel(out.cpu().numpy(), ref) < FP_RTOL
+    rows = np.array([5, -1, 40000, naw, 123], np.int32)
+    sub = torch.from_numpy(rows).cuda()
+    out = torch.full((5, 837), 7.0, device='cuda')
+    _lib.check(L.nipet_fprj(g.handle, out.data_ptr(), x.data_ptr(), sub.data_ptr(), 5, 0, st))
+    o = out.cpu().numpy()
+    assert np.all(o[[1, 3]] == 0)
+    assert _rel(o[[0, 2, 4]], O.fprj(params, img, rows[[0, 2, 4]])) < FP_RTOL
+    y = torch.rand((5, 837), device='cuda', generator=torch.Generator('cuda').manual_seed(21))
+    b_all = torch.empty_like(x)
+    _lib.check(L.nipet_bprj(g.handle, b_all.data_ptr(), y.data_ptr(), sub.data_ptr(), 5, st))
+    keep = torch.tensor([0, 2, 4], device='cuda')
+    yv, sv = y[keep].contiguous(), sub[keep].contiguous()
+    b_valid = torch.empty_like(x)
+    _lib.check(L.nipet_bprj(g.handle, b_valid.data_ptr(), yv.data_ptr(), sv.data_ptr(), 3, st))
+    assert float((b_all - b_valid).abs().max() / b_valid.abs().max()) < 1e-6
+    assert L.nipet_fprj(g.handle, out.data_ptr(), x.data_ptr(), None, 5, 0, st) == -1
+    assert b'nprj=5' in L.nipet_last_error()
+    assert L.nipet_bprj(g.handle, b_all.data_ptr(), y.data_ptr(), sub.data_ptr(), 0, st) == -1
+    assert L.nipet_fprj_b(g.handle, out.data_ptr(), x.data_ptr(), sub.data_ptr(), 5, 65, 0,
+                          st) == -1
+    assert b'batch 65' in L.nipet_last_error()
+    assert L.nipet_fprj(g.handle, out.data_ptr(), x.data_ptr(), sub.data_ptr(), 5, 2, st) == -1
+
+
 @pytest.fixture(scope='module')
 def params1(params):
     """The same scanner in span 1 (4084 sinograms, one LOR per bin)."""
