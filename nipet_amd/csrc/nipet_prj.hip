@@ -191,10 +191,11 @@ template <int NB, bool DIRECT, bool TWO>
 __device__ inline void fp_walk(FpLor &l, FpSum<NB> &acc, const float2 *__restrict__ sr,
                                const float *__restrict__ zs, int nch) {
   constexpr int PS = fp_ps<NB>() * NB;  // plane stride (floats)
+  constexpr int UNROLL = NB >= 4 ? 2 : 4;
   for (int j8 = 0; j8 < nch; j8 += SUBSUM) {
     const float *za = zs + l.ao + j8 * NB;
     const float *zb = zs + l.bo + j8 * NB;
-#pragma unroll(NB >= 4 ? 2 : 4)
+#pragma unroll UNROLL
     for (int jj = 0; jj < SUBSUM; jj++) {
       const float2 sl = sr[j8 + jj];
       float xa0[NB], xa1[NB];
@@ -446,10 +447,11 @@ template <int NB, int G, bool TWO>
 __device__ inline void fpg_walk(FpLor &l, FpSumG<NB, G> &acc, const float2 *__restrict__ sr,
                                 const float *__restrict__ zs, int nch) {
   constexpr int PS = fp_ps<NB>() * NB;  // plane stride (floats)
+  constexpr int UNROLL = NB >= 2 ? 2 : 4;
   for (int j8 = 0; j8 < nch; j8 += SUBSUM) {
     const float *za = zs + l.ao + j8 * NB;
     const float *zb = zs + l.bo + j8 * NB;
-#pragma unroll(NB >= 2 ? 2 : 4)
+#pragma unroll UNROLL
     for (int jj = 0; jj < SUBSUM; jj++) {
       const float2 sl = sr[j8 + jj];
       if constexpr (!TWO) {
