@@ -10,6 +10,7 @@
   this path has no scanner-file readers or scatter model).
 """
 import logging
+import os
 import time
 from collections.abc import Iterable
 from numbers import Real
@@ -18,7 +19,43 @@ import numpy as np
 
 log = logging.getLogger(__name__)
 
-__all__ = ['get_subsets14', 'psf_config', 'fwhm2sig', 'osemone', 'osem_prepare', 'osem_run']
+__all__ = ['get_subsets14', 'psf_config', 'fwhm2sig', 'osemone', 'osem_prepare', 'osem_run',
+           'RI_LUT', 'quant_factor']
+
+# Radioisotopes: half-life [s] and positron branching fraction. The reference
+# reads them from ``resources.riLUT`` of its un-vendored ``niftypet.ninst``
+# dependency (>= 0.12.0, ``setup.cfg:45``) at ``prj/mmrrec.py:309, 321, 327``;
+# the values here are the published decay data that table holds. The table
+# itself cannot be read here, so these constants are not pinned against it.
+RI_LUT = {
+    'Ge68': {'BF': 0.891, 'thalf': 270.9516 * 24 * 60 * 60},
+    'Ga68': {'BF': 0.891, 'thalf': 67.71 * 60},
+    'F18': {'BF': 0.967, 'thalf': 109.77120 * 60},
+    'O15': {'BF': 0.999, 'thalf': 122.2416},
+    'C11': {'BF': 0.998, 'thalf': 20.38 * 60},
+}
+
+
+def quant_factor(Cnt, hst, ncmp, decay_ref_time=None):
+    """Factor dcycrr * qf * qf_loc that ``osemone`` applies to the image
+    (``prj/mmrrec.py:308-331, 384-388``): with the normalisation components
+    ``ncmp`` and a histogram carrying ``t0`` and ``dur``, qf = ncmp['qf'] /
+    BF / dur, qf_loc = ncmp['qf_loc'], and with ``Cnt['DCYCRR']`` the decay
+    correction to ``decay_ref_time`` (default the frame start t0),
+    dcycrr = exp(l tref) l dur / (1 - exp(-l dur)), l = ln 2 / T_half.
+    Otherwise 1."""
+    if ncmp is None or 't0' not in hst or 'dur' not in hst:
+        return 1.0
+    iso = RI_LUT[Cnt['ISOTOPE']]
+    lmbd = np.log(2) / iso['thalf']
+    dur = float(hst['dur'])
+    if Cnt.get('DCYCRR'):
+        tref = hst['t0'] if decay_ref_time is None else decay_ref_time
+        dcycrr = np.exp(lmbd * tref) * lmbd * dur / (1 - np.exp(-lmbd * dur))
+    else:
+        dcycrr = 1.
+    qf = ncmp['qf'] / iso['BF'] / dur
+    return float(dcycrr * qf * ncmp['qf_loc'])
 
 
 def fwhm2sig(fwhm, voxsize=1.):
@@ -113,7 +150,15 @@ def osemone(datain, mumaps, hst, scanner_params, recmod=3, itr=4, fwhm=0., psf=N
     ones), ``randsino`` (additive; estimated from ``hst['fansums']``
     when None, as ``prj/mmrrec.py:244-250``, zero without fan-sums) and
     ``sctsino`` (additive; zero when None).
-    Returns a namedtuple ``(im, fpet, imsmo, fsmo, affine)`` with the E7 image.
+    As the reference, the image is scaled by ``quant_factor`` (decay correction
+    and quantification, when the normalisation components are known and ``hst``
+    carries ``t0`` / ``dur``) and, with ``fwhm > 0``, also returned smoothed by
+    a Gaussian of that FWHM [mm]. Returns a namedtuple ``(im, fpet, imsmo,
+    fsmo, affine)``: the E7 image, the reference's output file name (when
+    ``datain`` names the list-mode file, or ``fout`` is given; nothing is
+    written), the smoothed image, and None for the smoothed file name and the
+    affine (which needs the bed position from the DICOM headers). Writing
+    NIfTI files (``store_img``, ``store_itr``) is not part of this package.
     """
     from collections import namedtuple
 
@@ -121,6 +166,9 @@ def osemone(datain, mumaps, hst, scanner_params, recmod=3, itr=4, fwhm=0., psf=N
     from . import osem as _osem
     from . import mmrprj
     Cnt, txLUT = scanner_params['Cnt'], scanner_params['txLUT']
+    if store_img or store_itr:
+        raise NotImplementedError('osemone: writing NIfTI images (store_img / store_itr) is not '
+                                  'part of nipet_amd; save RecOut.im with a NIfTI library')
 
     def dev(s):
         if s is None:
@@ -141,13 +189,15 @@ def osemone(datain, mumaps, hst, scanner_params, recmod=3, itr=4, fwhm=0., psf=N
         mus = mmrimg.convert2dev(np.asarray(muo) + np.asarray(muh), Cnt)
         asng = mmrprj.frwd_prj(mus, scanner_params, dev_out=True, attenuation=True)
         asng = np.asarray(asng.cpu() if hasattr(asng, 'cpu') else asng)
+    ncmp = normcomp
+    if ncmp is None and datain.get('nrm_bf') and (datain.get('nrm_ima') or datain.get('nrm_dcm')):
+        from ..mmrnorm import get_components
+        ncmp = get_components(datain, Cnt)[0]
     if normsino is not None:
         nsng = dev(normsino)
-    elif normcomp is not None or (datain.get('nrm_bf') and
-                                  (datain.get('nrm_ima') or datain.get('nrm_dcm'))):
+    elif ncmp is not None:
         # prj/mmrrec.py:209-215: the normalisation sinogram from its components
-        from ..mmrnorm import NormSino, get_components
-        ncmp = normcomp if normcomp is not None else get_components(datain, Cnt)[0]
+        from ..mmrnorm import NormSino
         nsng = NormSino(scanner_params)(ncmp, hst['buckets'])
     else:
         nsng = np.ones(psng.shape, np.float32)
@@ -165,6 +215,27 @@ def osemone(datain, mumaps, hst, scanner_params, recmod=3, itr=4, fwhm=0., psf=N
                                mask_radius=mask_radius, psf=psfkernel)
     img = _osem.osem_run(state, itr)
     log.info('recon time: %.3g', time.time() - stime)
+    qscale = quant_factor(Cnt, hst, ncmp, decay_ref_time)
+    if qscale != 1.0:
+        log.info('applying decay correction and quantification factor %r', qscale)
+        img *= qscale
     im = mmrimg.convert2e7(img, Cnt)
+    # output file name of the reference (prj/mmrrec.py:175-186, 406-413)
+    opth = outpath if outpath is not None else (
+        os.path.join(datain['corepath'], 'reconstructed') if datain.get('corepath') else '')
+    fpet = None
+    if fout is not None:
+        fpet = os.path.join(opth, os.path.basename(fout).split('.')[0] +
+                            f'_itr{state.iterations}{fcomment}.nii.gz')
+    elif datain.get('lm_bf') and 't0' in hst and 't1' in hst:
+        fpet = os.path.join(opth, os.path.basename(datain['lm_bf']).split('.')[0] +
+                            f"{frmno}_t{hst['t0']}-{hst['t1']}sec_itr{state.iterations}"
+                            f"{fcomment}.nii.gz")
+    im_smo = None
+    if fwhm > 0:
+        # prj/mmrrec.py:420-421
+        from scipy import ndimage as ndi
+        im_smo = ndi.gaussian_filter(im, fwhm2sig(fwhm, voxsize=Cnt['SZ_VOXY'] * 10),
+                                     mode='mirror')
     RecOut = namedtuple('RecOut', 'im, fpet, imsmo, fsmo, affine')
-    return RecOut(im, None, None, None, None)
+    return RecOut(im, fpet, im_smo, None, None)

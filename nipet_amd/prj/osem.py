@@ -12,12 +12,16 @@ Differences in structure (not in arithmetic) from the reference:
   back-projected in its first update, in the same launch as the ratio
   (``nipet_osem_subset_sens``: two images per traversal of the rays), not in a
   separate pass before the loop (``prj/mmrrec.py:291-298``). The images are
-  the same; ``OsemState.sensitivity()`` completes the missing ones on demand.
+  the same; reading ``OsemState.sens`` completes any missing ones.
 """
+import logging
+
 import numpy as np
 
 from .. import _lib
 from ..geom import get_geom
+
+log = logging.getLogger(__name__)
 
 __all__ = ['OsemState', 'osem_prepare', 'osem_run', 'sensitivity', 'cylinder_mask']
 
@@ -245,9 +249,13 @@ def osem_run(state, niter, img0=None, check_counts=True):
     torch = _torch()
     img = torch.ones((320, 320, 127), dtype=torch.float32, device='cuda') if img0 is None \
         else _dev(img0, torch.float32).clone()
-    for _ in range(niter):
+    state.iterations = 0
+    for k in range(niter):
         osem_iteration(state, img)
+        state.iterations = k + 1
         if check_counts and float(torch.nansum(img)) < 0.1:
+            log.warning('it seems there is not enough true data to render reasonable image')
+            state.iterations = k      # as prj/mmrrec.py:357-361
             break
     return img
 

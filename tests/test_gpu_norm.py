@@ -81,6 +81,24 @@ def test_get_norm_sino_layouts_and_osemone(torch, params):
     rel = lambda x, y: np.linalg.norm((x - y).ravel()) / np.linalg.norm(y.ravel())  # noqa: E731
     assert rel(a.im, b.im) <= 1e-6
     assert rel(a.im, c.im) > 1e-3
+    # with the frame times in the histogram the image carries the reference's
+    # quantification factor qf / BF / dur * qf_loc (prj/mmrrec.py:324-331, 388),
+    # and fwhm > 0 adds the smoothed image (prj/mmrrec.py:420-421)
+    from nipet_amd.prj.mmrrec import quant_factor
+    cq = dict(comp, qf=1.25e5, qf_loc=0.205)
+    hq = {'psino': psino, 'buckets': bk, 't0': 30, 't1': 630, 'dur': 600}
+    d = osemone({'lm_bf': '/data/LM/scan.bf', 'corepath': '/data'}, mus, hq, params,
+                recmod=0, itr=1, normcomp=cq, fwhm=4.0)
+    q = quant_factor(params['Cnt'], hq, cq)
+    assert abs(q - 1.25e5 / 0.967 / 600 * 0.205) <= 1e-9 * q
+    assert rel(d.im, a.im * np.float32(q)) <= 1e-6
+    assert d.fpet == '/data/reconstructed/scan_t30-630sec_itr1.nii.gz'
+    assert d.imsmo.shape == d.im.shape
+    tv = lambda x: np.abs(np.diff(x, axis=2)).sum()  # noqa: E731
+    assert tv(d.imsmo) < tv(d.im)
+    assert abs(float(d.imsmo.sum(dtype=np.float64)) / float(d.im.sum(dtype=np.float64)) - 1) < 1e-4
+    with pytest.raises(NotImplementedError):
+        osemone({}, mus, hq, params, recmod=0, itr=1, store_img=True)
 
 
 def test_norm_bad_arguments(torch, params):
