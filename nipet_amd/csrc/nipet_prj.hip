@@ -681,12 +681,16 @@ __global__ __launch_bounds__(fpg_threads<G>()) void k_fprj_g(GeomView g, const f
 // segments and walks all ring differences; for a given d, w0 changes between
 // consecutive segments only when a z boundary lies between their starts, so
 // the table reads and sums are kept in registers and reloaded only on change.
-// The pieces of (segment, d) are computed by lane d (times the 3D length
-// factor of d) and broadcast with readlane. va / vb are read from an LDS copy
-// of the sinogram row through per-(d, r0) index tables whose padding entries
-// point at a zero row, so clamped indices read zeros. No atomics inside a ray:
-// each wave flushes its segments through a private LDS buffer with one
-// coalesced global float atomic per (segment, plane, replicate).
+// Lane d computes the pieces of each segment of the run (times the 3D length
+// factor sf of d) and one run word: w0 of the first segment, which segments
+// hold a z boundary, and the w0 step after each segment. Without a boundary a
+// segment has the single piece L sf (L: its length, known to every lane), so
+// the d loop broadcasts one word per run and the pieces only for boundary
+// segments (about one in four). va / vb are read from an LDS copy of the
+// sinogram row through per-(d, r0) index tables whose padding entries point
+// at a zero row, so clamped indices read zeros. No atomics inside a ray: each
+// wave flushes its segments through a private LDS buffer with one coalesced
+// global float atomic per (segment, plane, replicate).
 // min(max(v, lo), hi): one v_med3_i32
 __device__ inline int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
@@ -754,7 +758,7 @@ __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g
   for (int g0 = wave * SPW; g0 < nseg; g0 += NWAVE * SPW) {
     // lane d: base plane and pieces (times the 3D factor) of each segment
     float P1[SPW], P2[SPW], P3[SPW], L[SPW];
-    int W0[SPW], C[SPW];
+    int RW = 0;  // the run word of lane d
     int wlast = -1;
     static_for<SPW>([&](auto sc) {
       constexpr int s = decltype(sc)::value;
@@ -768,9 +772,7 @@ __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g
         sg.colz = 0;
       }
       L[s] = sg.len;
-      C[s] = sg.colz;
       P1[s] = P2[s] = P3[s] = 0.f;
-      W0[s] = 0;
       if (lane >= 1 && lane <= dmax) {
         double w0 = floor(0.5 + m_lane * sg.s);
         double tb = (w0 + 0.5) * inv_lane;
@@ -782,11 +784,15 @@ __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g
         P3[s] = (sg.len - c2) * sf_lane;
         // base plane (bits 0-7), pieces 2 / 3 non-zero (bits 8, 9), base plane
         // differs from the previous segment's: the LOR values change (bit 10)
-        W0[s] = (int)w0 | ((int)(P2[s] != 0.f) << 8) | ((int)(P3[s] != 0.f) << 9);
-        if constexpr (NB > 1) {
-          W0[s] |= (int)(s == 0 || (int)w0 != wlast) << 10;
-          wlast = (int)w0;
-        }
+        // run word: base plane of the first segment (bits 0-7), a z boundary
+        // inside segment s (bit 8 + s), base-plane step from segment s to
+        // s + 1 (bits 16 + 2s, 2 bits; at most 2 by the geometry check)
+        if (s == 0)
+          RW = (int)w0;
+        else if (k < nseg)
+          RW |= min((int)w0 - wlast, 3) << (16 + 2 * (s - 1));
+        RW |= (int)(P2[s] != 0.f || P3[s] != 0.f) << (8 + s);
+        wlast = (int)w0;
       }
     });
     float acc0[SPW][NB], acc1[SPW][NB];
@@ -798,118 +804,75 @@ __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g
         acc1[s][b] = 0.f;
       }
     });
-    if constexpr (NB == 1) {
-      // one image: parity by branch, flags tested one by one (fewer
-      // registers, three workgroups per CU)
-      for (int d = 1; d <= dmax; d++) {
-        const int rb = bp_row(d) + 2;
-        float S1[NB], S3[NB], Te[NB], To[NB];
-        int wprev = -1000;
-        static_for<SPW>([&](auto sc) {
-          constexpr int s = decltype(sc)::value;
-          const int wv = __builtin_amdgcn_readlane(W0[s], d);
-          const int w0 = wv & 0xff;
-          if (w0 != wprev) {  // wave-uniform: new base plane -> reload the LOR values
-            wprev = w0;
-            const int h = w0 >> 1, e = (2 * d - w0) >> 1;
-            const int ka = rb + clampi(q - h, -1, 65 - d);
-            const int kb = rb + clampi(q - e, -2, 64 - d);
-            const int ta = pa[ka], tbb = pb[kb];
-            float X[NB], Y[NB], Xb[NB], Yb[NB];
-            ldv<NB>(X, srow + (ta & 0xffff) * NB);
-            ldv<NB>(Y, srow + (ta >> 16) * NB);
-            ldv<NB>(Xb, srow + (tbb & 0xffff) * NB);
-            ldv<NB>(Yb, srow + (tbb >> 16) * NB);
-  #pragma unroll
-            for (int b = 0; b < NB; b++) {
-              S1[b] = X[b] + Xb[b];
-              S3[b] = Y[b] + Yb[b];
-              Te[b] = X[b] + Yb[b];
-              To[b] = Y[b] + Xb[b];
-            }
-          }
-          const float p1 = rdlane(P1[s], d);
-          if (w0 & 1) {
-  #pragma unroll
-            for (int b = 0; b < NB; b++) acc1[s][b] = fmaf(p1, S1[b], acc1[s][b]);
-          } else {
-  #pragma unroll
-            for (int b = 0; b < NB; b++) acc0[s][b] = fmaf(p1, S1[b], acc0[s][b]);
-          }
-          if (wv & 0x100) {  // a z boundary inside the segment (zero pieces are skipped)
-            const float p2 = rdlane(P2[s], d);
-            if (w0 & 1) {
-  #pragma unroll
-              for (int b = 0; b < NB; b++) acc0[s][b] = fmaf(p2, To[b], acc0[s][b]);
-            } else {
-  #pragma unroll
-              for (int b = 0; b < NB; b++) acc1[s][b] = fmaf(p2, Te[b], acc1[s][b]);
-            }
-          }
-          if (wv & 0x200) {
-            const float p3 = rdlane(P3[s], d);
-            if (w0 & 1) {
-  #pragma unroll
-              for (int b = 0; b < NB; b++) acc1[s][b] = fmaf(p3, S3[b], acc1[s][b]);
-            } else {
-  #pragma unroll
-              for (int b = 0; b < NB; b++) acc0[s][b] = fmaf(p3, S3[b], acc0[s][b]);
-            }
-          }
-        });
-      }
-    } else {
-      // batches: one flag test per step, scalar-selected coefficients
-      for (int d = 1; d <= dmax; d++) {
-        const int rb = bp_row(d) + 2;
-        float S1[NB], S3[NB], Te[NB], To[NB];
-        static_for<SPW>([&](auto sc) {
-          constexpr int s = decltype(sc)::value;
-          const int wv = __builtin_amdgcn_readlane(W0[s], d);
-          const float p1 = rdlane(P1[s], d);
-          const bool odd = wv & 1;
-          if (wv & 0x700) {  // wave-uniform, about one step in three
-            if (wv & 0x400) {  // new base plane: reload the LOR values
-              const int w0 = wv & 0xff;
-              const int h = w0 >> 1, e = (2 * d - w0) >> 1;
-              const int ka = rb + clampi(q - h, -1, 65 - d);
-              const int kb = rb + clampi(q - e, -2, 64 - d);
-              const int ta = pa[ka], tbb = pb[kb];
-              float X[NB], Y[NB], Xb[NB], Yb[NB];
-              ldv<NB>(X, srow + (ta & 0xffff) * NB);
-              ldv<NB>(Y, srow + (ta >> 16) * NB);
-              ldv<NB>(Xb, srow + (tbb & 0xffff) * NB);
-              ldv<NB>(Yb, srow + (tbb >> 16) * NB);
-  #pragma unroll
-              for (int b = 0; b < NB; b++) {
-                S1[b] = X[b] + Xb[b];
-                S3[b] = Y[b] + Yb[b];
-                Te[b] = X[b] + Yb[b];
-                To[b] = Y[b] + Xb[b];
-              }
-            }
-            if (wv & 0x300) {  // pieces 2 and 3 (z boundaries inside the segment)
-              const float p2 = (wv & 0x100) ? rdlane(P2[s], d) : 0.f;
-              const float p3 = (wv & 0x200) ? rdlane(P3[s], d) : 0.f;
-              const float a2 = odd ? p2 : 0.f, a3 = odd ? 0.f : p3;
-              const float c2 = odd ? 0.f : p2, c3 = odd ? p3 : 0.f;
-  #pragma unroll
-              for (int b = 0; b < NB; b++) {
-                acc0[s][b] = fmaf(a2, To[b], fmaf(a3, S3[b], acc0[s][b]));
-                acc1[s][b] = fmaf(c2, Te[b], fmaf(c3, S3[b], acc1[s][b]));
-              }
-            }
-          }
-          // piece 1: plane 2q (even base plane) or 2q+1 (odd); the coefficients
-          // are selected on the scalar unit, both FMAs always run
-          const float a1 = odd ? 0.f : p1, c1 = odd ? p1 : 0.f;
-  #pragma unroll
+    // The run word of d gives the base plane, the segments that hold a z
+    // boundary and the base-plane steps. Every segment first takes the single
+    // piece L * sf on plane 2q (even base plane) or 2q + 1: two FMAs per image
+    // with the parity folded into the coefficients (sf * S1 or 0), no
+    // broadcast. A segment with a boundary then swaps that piece for its
+    // three broadcast pieces, the parity as SGPR bit masks.
+    for (int d = 1; d <= dmax; d++) {
+      const int rb = bp_row(d) + 2;
+      const int rw = __builtin_amdgcn_readlane(RW, d);
+      const float sf = rdlane(sf_lane, d);
+      int w0 = rw & 0xff;
+      float S1[NB], S3[NB], Te[NB], To[NB], Ae[NB], Ao[NB];
+      auto reload = [&]() {
+        const int h = w0 >> 1, e = (2 * d - w0) >> 1;
+        const int ka = rb + clampi(q - h, -1, 65 - d);
+        const int kb = rb + clampi(q - e, -2, 64 - d);
+        const int ta = pa[ka], tbb = pb[kb];
+        float X[NB], Y[NB], Xb[NB], Yb[NB];
+        ldv<NB>(X, srow + (ta & 0xffff) * NB);
+        ldv<NB>(Y, srow + (ta >> 16) * NB);
+        ldv<NB>(Xb, srow + (tbb & 0xffff) * NB);
+        ldv<NB>(Yb, srow + (tbb >> 16) * NB);
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+          S1[b] = X[b] + Xb[b];
+          S3[b] = Y[b] + Yb[b];
+          Te[b] = X[b] + Yb[b];
+          To[b] = Y[b] + Xb[b];
+          const float a = sf * S1[b];
+          Ae[b] = (w0 & 1) ? 0.f : a;
+          Ao[b] = (w0 & 1) ? a : 0.f;
+        }
+      };
+      reload();
+      static_for<SPW>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+          acc0[s][b] = fmaf(L[s], Ae[b], acc0[s][b]);
+          acc1[s][b] = fmaf(L[s], Ao[b], acc1[s][b]);
+        }
+        if (rw & (1 << (8 + s))) {  // wave-uniform, about one segment in four
+          const int mo = (w0 << 31) >> 31;  // all ones for an odd base plane
+          const int p1 = __builtin_amdgcn_readlane(__float_as_int(P1[s]), d);
+          const int p2 = __builtin_amdgcn_readlane(__float_as_int(P2[s]), d);
+          const int p3 = __builtin_amdgcn_readlane(__float_as_int(P3[s]), d);
+          const float a1 = __int_as_float(p1 & ~mo), c1 = __int_as_float(p1 & mo);
+          const float a2 = __int_as_float(p2 & mo), c2 = __int_as_float(p2 & ~mo);
+          const float a3 = __int_as_float(p3 & ~mo), c3 = __int_as_float(p3 & mo);
+#pragma unroll
           for (int b = 0; b < NB; b++) {
+            acc0[s][b] = fmaf(-L[s], Ae[b], acc0[s][b]);
+            acc1[s][b] = fmaf(-L[s], Ao[b], acc1[s][b]);
             acc0[s][b] = fmaf(a1, S1[b], acc0[s][b]);
+            acc0[s][b] = fmaf(a2, To[b], acc0[s][b]);
+            acc0[s][b] = fmaf(a3, S3[b], acc0[s][b]);
             acc1[s][b] = fmaf(c1, S1[b], acc1[s][b]);
+            acc1[s][b] = fmaf(c2, Te[b], acc1[s][b]);
+            acc1[s][b] = fmaf(c3, S3[b], acc1[s][b]);
           }
-        });
-      }
+        }
+        if constexpr (s + 1 < SPW) {
+          const int dl = (rw >> (16 + 2 * s)) & 3;
+          if (dl) {  // wave-uniform: the next segment starts on a new base plane
+            w0 += dl;
+            reload();
+          }
+        }
+      });
     }
     // flush the wave's segments through its LDS buffer (wave-local, no barrier)
     static_for<SPW>([&](auto sc) {
@@ -922,7 +885,7 @@ __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g
       }
       __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the wave's LDS writes landed
       __builtin_amdgcn_wave_barrier();
-      const int colz = __builtin_amdgcn_readfirstlane(C[s]);
+      const int colz = g.seg[beg + g0 + s].colz;
       for (int i = lane; i < NZ * NB; i += 64) {
         const int z = i / NB, b = i - z * NB;
         const float v = wb[i];
