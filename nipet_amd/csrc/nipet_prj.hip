@@ -845,31 +845,35 @@ __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g
           acc0[s][b] = fmaf(L[s], Ae[b], acc0[s][b]);
           acc1[s][b] = fmaf(L[s], Ao[b], acc1[s][b]);
         }
-        if (rw & (1 << (8 + s))) {  // wave-uniform, about one segment in four
-          const int mo = (w0 << 31) >> 31;  // all ones for an odd base plane
-          const int p1 = __builtin_amdgcn_readlane(__float_as_int(P1[s]), d);
-          const int p2 = __builtin_amdgcn_readlane(__float_as_int(P2[s]), d);
-          const int p3 = __builtin_amdgcn_readlane(__float_as_int(P3[s]), d);
-          const float a1 = __int_as_float(p1 & ~mo), c1 = __int_as_float(p1 & mo);
-          const float a2 = __int_as_float(p2 & mo), c2 = __int_as_float(p2 & ~mo);
-          const float a3 = __int_as_float(p3 & ~mo), c3 = __int_as_float(p3 & mo);
+        // one scalar test for the common step (no boundary, same base plane)
+        constexpr int EV = (1 << (8 + s)) | (s + 1 < SPW ? 3 << (16 + 2 * s) : 0);
+        if (rw & EV) {
+          if (rw & (1 << (8 + s))) {  // wave-uniform, about one segment in four
+            const int mo = (w0 << 31) >> 31;  // all ones for an odd base plane
+            const int p1 = __builtin_amdgcn_readlane(__float_as_int(P1[s]), d);
+            const int p2 = __builtin_amdgcn_readlane(__float_as_int(P2[s]), d);
+            const int p3 = __builtin_amdgcn_readlane(__float_as_int(P3[s]), d);
+            const float a1 = __int_as_float(p1 & ~mo), c1 = __int_as_float(p1 & mo);
+            const float a2 = __int_as_float(p2 & mo), c2 = __int_as_float(p2 & ~mo);
+            const float a3 = __int_as_float(p3 & ~mo), c3 = __int_as_float(p3 & mo);
 #pragma unroll
-          for (int b = 0; b < NB; b++) {
-            acc0[s][b] = fmaf(-L[s], Ae[b], acc0[s][b]);
-            acc1[s][b] = fmaf(-L[s], Ao[b], acc1[s][b]);
-            acc0[s][b] = fmaf(a1, S1[b], acc0[s][b]);
-            acc0[s][b] = fmaf(a2, To[b], acc0[s][b]);
-            acc0[s][b] = fmaf(a3, S3[b], acc0[s][b]);
-            acc1[s][b] = fmaf(c1, S1[b], acc1[s][b]);
-            acc1[s][b] = fmaf(c2, Te[b], acc1[s][b]);
-            acc1[s][b] = fmaf(c3, S3[b], acc1[s][b]);
+            for (int b = 0; b < NB; b++) {
+              acc0[s][b] = fmaf(-L[s], Ae[b], acc0[s][b]);
+              acc1[s][b] = fmaf(-L[s], Ao[b], acc1[s][b]);
+              acc0[s][b] = fmaf(a1, S1[b], acc0[s][b]);
+              acc0[s][b] = fmaf(a2, To[b], acc0[s][b]);
+              acc0[s][b] = fmaf(a3, S3[b], acc0[s][b]);
+              acc1[s][b] = fmaf(c1, S1[b], acc1[s][b]);
+              acc1[s][b] = fmaf(c2, Te[b], acc1[s][b]);
+              acc1[s][b] = fmaf(c3, S3[b], acc1[s][b]);
+            }
           }
-        }
-        if constexpr (s + 1 < SPW) {
-          const int dl = (rw >> (16 + 2 * s)) & 3;
-          if (dl) {  // wave-uniform: the next segment starts on a new base plane
-            w0 += dl;
-            reload();
+          if constexpr (s + 1 < SPW) {
+            const int dl = (rw >> (16 + 2 * s)) & 3;
+            if (dl) {  // wave-uniform: the next segment starts on a new base plane
+              w0 += dl;
+              reload();
+            }
           }
         }
       });
