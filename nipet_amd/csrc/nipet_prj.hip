@@ -702,6 +702,7 @@ __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g
                                                      const float *__restrict__ sino,
                                                      const int32_t *__restrict__ subs, Batch bt) {
   constexpr int SPW = bp_spw<NB>();
+  static_assert(SPW <= 8, "the run word holds the boundary bits and steps of 8 segments");
   constexpr int NWAVE = BP_THREADS / 64;
   __shared__ __align__(16) float srow[(MAXNSN / NB + 1) * NB];   // sinogram row + zero row
   __shared__ int32_t pa[BP_TABN];  // (idx_a[k-1] << 16) | idx_a[k]
