@@ -30,7 +30,9 @@ constexpr int MAXNB = 4;          // replicates per kernel instance (larger batc
 constexpr int BP_MAXD = 63;       // largest ring difference of the BP tables
 
 // BP table row d holds r0 = -2 .. 65-d (two zero pads on both sides); row offset:
-__host__ __device__ constexpr int bp_row(int d) { return 68 * d - d * (d - 1) / 2; }
+// Row d of the BP index tables covers r0 = -d-1 .. 64 (d + 66 entries), every
+// index the gather can form for ring difference d, so no clamping is needed.
+__host__ __device__ constexpr int bp_row(int d) { return 66 * d + d * (d - 1) / 2; }
 constexpr int BP_TABN = bp_row(BP_MAXD + 1);
 
 struct GeomView {
@@ -687,12 +689,12 @@ __global__ __launch_bounds__(fpg_threads<G>()) void k_fprj_g(GeomView g, const f
 // segment has the single piece L sf (L: its length, known to every lane), so
 // the d loop broadcasts one word per run and the pieces only for boundary
 // segments (about one in four). va / vb are read from an LDS copy of the
-// sinogram row through per-(d, r0) index tables whose padding entries point
-// at a zero row, so clamped indices read zeros. No atomics inside a ray: each
-// wave flushes its segments through a private LDS buffer with one coalesced
-// global float atomic per (segment, plane, replicate).
-// min(max(v, lo), hi): one v_med3_i32
-__device__ inline int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+// sinogram row through per-(d, r0) tables of byte offsets into that row; a
+// table row spans every r0 the gather can form (-d-1 .. 64), with the entries
+// outside 0 .. 63-d pointing at a zero row, so a reload is one add per table
+// and no clamp. No atomics inside a ray: each wave flushes its segments
+// through a private LDS buffer with one coalesced global float atomic per
+// (segment, plane, replicate).
 
 template <int NB>
 constexpr int bp_spw() { return NB == 1 ? 8 : (NB == 2 ? 4 : 6); }  // measured per width
@@ -705,8 +707,6 @@ __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g
   static_assert(SPW <= 8, "the run word holds the boundary bits and steps of 8 segments");
   constexpr int NWAVE = BP_THREADS / 64;
   __shared__ __align__(16) float srow[(MAXNSN / NB + 1) * NB];   // sinogram row + zero row
-  __shared__ int32_t pa[BP_TABN];  // (idx_a[k-1] << 16) | idx_a[k]
-  __shared__ int32_t pb[BP_TABN];  // (idx_b[k+1] << 16) | idx_b[k]
   __shared__ __align__(16) float wbuf[NWAVE][ZSTRIDE * NB];
   __shared__ int dmax_s;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -718,15 +718,18 @@ __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g
   const int beg = g.off[aw], nseg = g.off[aw + 1] - beg;
   const double atn = g.atn[aw];
 
-  // LDS: sinogram row, zero row at index nsn, LOR -> sinogram index tables
+  // LDS: sinogram row, zero row at index nsn, and the LOR -> sinogram tables
+  // (ta: LOR (r0, d), tb: its mirror) as byte offsets into the row; entries
+  // outside 0 <= r0 <= 63 - d point at the zero row
   for (int i = tid; i < (nsn + 1) * NB; i += BP_THREADS) {
     const int s = i / NB, b = i - s * NB;
     srow[i] = s < nsn ? in[(size_t)s * nbt + b] : 0.f;
   }
-  __shared__ int16_t ia_tab[BP_TABN], ib_tab[BP_TABN];
+  __shared__ uint16_t ta[BP_TABN], tb[BP_TABN];
+  const uint16_t zoff = (uint16_t)(nsn * NB * 4);
   for (int i = tid; i < BP_TABN; i += BP_THREADS) {
-    ia_tab[i] = (int16_t)nsn;
-    ib_tab[i] = (int16_t)nsn;
+    ta[i] = zoff;
+    tb[i] = zoff;
   }
   if (tid == 0) dmax_s = 0;
   __syncthreads();
@@ -734,18 +737,13 @@ __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g
     float2 z = g.li_z[li];
     short2 sn = g.li_sn[li];
     int r0 = (int)floorf(z.x) >> 1, d = (int)z.y >> 1;
-    int k = bp_row(d) + 2 + r0;
-    ia_tab[k] = sn.x;
-    if (d > 0) ib_tab[k] = sn.y;
+    int k = bp_row(d) + d + 1 + r0;
+    ta[k] = (uint16_t)(sn.x * NB * 4);
+    if (d > 0) tb[k] = (uint16_t)(sn.y * NB * 4);
     atomicMax(&dmax_s, d);
   }
   __syncthreads();
-  for (int i = tid; i < BP_TABN; i += BP_THREADS) {
-    const int prev = i > 0 ? ia_tab[i - 1] : nsn, next = i + 1 < BP_TABN ? ib_tab[i + 1] : nsn;
-    pa[i] = (prev << 16) | (ia_tab[i] & 0xffff);
-    pb[i] = (next << 16) | (ib_tab[i] & 0xffff);
-  }
-  __syncthreads();
+  const char *srb = (const char *)srow;
   const int dmax = dmax_s;
   const double m_lane = 2.0 * lane / atn;
   const double inv_lane = lane > 0 ? atn / (2.0 * lane) : 0.0;
@@ -753,7 +751,7 @@ __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g
   const int q = lane;
   // direct planes: plane 2q <- L * va0[q]
   float X0[NB];
-  ldv<NB>(X0, srow + (ia_tab[bp_row(0) + 2 + q] & 0xffff) * NB);
+  ldv<NB>(X0, (const float *)(srb + ta[bp_row(0) + 1 + q]));
   float *wb = wbuf[wave];
 
   for (int g0 = wave * SPW; g0 < nseg; g0 += NWAVE * SPW) {
@@ -775,7 +773,8 @@ __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g
       L[s] = sg.len;
       P1[s] = P2[s] = P3[s] = 0.f;
       if (lane >= 1 && lane <= dmax) {
-        double w0 = floor(0.5 + m_lane * sg.s);
+        // base plane, kept in 0 .. 2d so the table indices stay inside row d
+        double w0 = fmin(fmax(floor(0.5 + m_lane * sg.s), 0.0), 2.0 * lane);
         double tb = (w0 + 0.5) * inv_lane;
         float q1 = (float)(tb - sg.s), q2 = (float)(tb + inv_lane - sg.s);
         float p1 = __builtin_amdgcn_fmed3f(q1, 0.f, sg.len);
@@ -812,21 +811,21 @@ __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g
     // broadcast. A segment with a boundary then swaps that piece for its
     // three broadcast pieces, the parity as SGPR bit masks.
     for (int d = 1; d <= dmax; d++) {
-      const int rb = bp_row(d) + 2;
+      const int rb = bp_row(d) + d + 1;  // entry of r0 = 0 in row d
       const int rw = __builtin_amdgcn_readlane(RW, d);
       const float sf = rdlane(sf_lane, d);
       int w0 = rw & 0xff;
       float S1[NB], S3[NB], Te[NB], To[NB], Ae[NB], Ao[NB];
       auto reload = [&]() {
+        // 0 <= w0 <= 2d (the run word's start and steps keep it there) puts
+        // q - h and q - e in -d .. 63, inside row d
         const int h = w0 >> 1, e = (2 * d - w0) >> 1;
-        const int ka = rb + clampi(q - h, -1, 65 - d);
-        const int kb = rb + clampi(q - e, -2, 64 - d);
-        const int ta = pa[ka], tbb = pb[kb];
+        const int ka = rb - h + q, kb = rb - e + q;
         float X[NB], Y[NB], Xb[NB], Yb[NB];
-        ldv<NB>(X, srow + (ta & 0xffff) * NB);
-        ldv<NB>(Y, srow + (ta >> 16) * NB);
-        ldv<NB>(Xb, srow + (tbb & 0xffff) * NB);
-        ldv<NB>(Yb, srow + (tbb >> 16) * NB);
+        ldv<NB>(X, (const float *)(srb + ta[ka]));
+        ldv<NB>(Y, (const float *)(srb + ta[ka - 1]));
+        ldv<NB>(Xb, (const float *)(srb + tb[kb]));
+        ldv<NB>(Yb, (const float *)(srb + tb[kb + 1]));
 #pragma unroll
         for (int b = 0; b < NB; b++) {
           S1[b] = X[b] + Xb[b];
