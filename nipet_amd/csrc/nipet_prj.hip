@@ -810,8 +810,8 @@ __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g
     // with the parity folded into the coefficients (sf * S1 or 0), no
     // broadcast. A segment with a boundary then swaps that piece for its
     // three broadcast pieces, the parity as SGPR bit masks.
-    for (int d = 1; d <= dmax; d++) {
-      const int rb = bp_row(d) + d + 1;  // entry of r0 = 0 in row d
+    // rb: entry of r0 = 0 in row d, bp_row(d) + d + 1, carried from d to d + 1
+    for (int d = 1, rb = bp_row(1) + 2; d <= dmax; rb += 67 + d, d++) {
       const int rw = __builtin_amdgcn_readlane(RW, d);
       const float sf = rdlane(sf_lane, d);
       int w0 = rw & 0xff;
