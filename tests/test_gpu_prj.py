@@ -195,6 +195,42 @@ def test_ragged_and_invalid_rows_through_c_abi(torch, params, img):
     assert L.nipet_fprj(g.handle, out.data_ptr(), x.data_ptr(), sub.data_ptr(), 5, 2, st) == -1
 
 
+def test_steepest_bins_vs_oracle(torch, params):
+    """The bins whose longest segment is largest against the chord (where the
+    steepest LORs cross two z boundaries in one segment, so the BP's base plane
+    steps by 2 and its three-piece path runs): FP and BP against the oracle,
+    and a check that such two-boundary segments exist in the selection."""
+    from oracle import oracle as O
+
+    import nipet_amd as nip
+    from nipet_amd.geom import get_geom
+    g = get_geom(params)
+    off, _, _, ln = g.tx_lists()
+    naw = params['txLUT']['Naw']
+    crs = params['txLUT']['crs'].astype(np.float64)
+    c = params['txLUT']['s2c'].astype(np.int64)
+    pa = 0.5 * (crs[c[:, 0], :2] + crs[c[:, 0], 2:])
+    pb = 0.5 * (crs[c[:, 1], :2] + crs[c[:, 1], 2:])
+    atn = np.hypot(*(pb - pa).T)
+    ok = np.diff(off) > 0
+    lmax = np.zeros(naw)
+    lmax[ok] = np.maximum.reduceat(ln.astype(np.float64), off[:-1][ok])
+    dmax = int(np.abs(np.diff(params['axLUT']['li2rno'].astype(np.int64), axis=1)).max())
+    ratio = 2.0 * dmax * lmax / atn          # z-boundary spacings per longest segment
+    rows = np.sort(np.argsort(-ratio)[:48]).astype(np.int32)
+    assert ratio[rows].max() > 1.0, ratio[rows].max()   # two boundaries fit in one segment
+    # these chords graze the edge of the field of view: a random image covers it
+    xr = np.random.default_rng(30).random((320, 320, 127), dtype=np.float32)
+    fp = nip.frwd_prj(xr, params, isub=rows, dev_out=True, fullsino_out=False)
+    ref = O.fprj(params, xr, rows)
+    assert np.abs(ref).max() > 0
+    assert _rel(fp, ref) < FP_RTOL
+    y = np.random.default_rng(31).random((len(rows), 837)).astype(np.float32)
+    bimg = nip.back_prj(y, params, isub=rows, dev_out=True)
+    ref = O.bprj(params, y.astype(np.float64), rows)
+    assert np.abs(bimg.astype(np.float64) - ref).max() / np.abs(ref).max() < 1e-5
+
+
 @pytest.fixture(scope='module')
 def params1(params):
     """The same scanner in span 1 (4084 sinograms, one LOR per bin)."""
