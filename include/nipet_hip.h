@@ -162,6 +162,28 @@ int nipet_osem_subset_b(const NipetGeom *g, float *img, const uint16_t *prompts,
                         const uint8_t *mask, const int32_t *subs, int nprj, int nb,
                         float *work_sino, float *work_img, void *stream);
 
+/* One OSEM iteration over all subsets: the extension call petprj.osem
+ * (osem_rec, prj/src/prj_module.cu:613-839; osem, prj/src/recon.cu:318-521)
+ * with the reference's arguments:
+ *   rho = (r + s) / (a n) (0 where |a n| < 1e-6), formed once per call
+ *   (recon.cu:381-398); then for subsets i = 0 .. nsub-1 in order the update of
+ *   nipet_osem_subset with the sensitivity image sens + i * 320*320*127 (with
+ *   resolution modelling, krnl[0] >= 0, that of nipet_osem_subset_rm_b with
+ *   the convolved sensitivity image, recon.cu:441-445, 468-495).
+ * img: device float image (in/out); prompts: device uint16 [naw][nsn];
+ * r, s (either may be NULL = zero), n, a: device float [naw][nsn];
+ * subs: HOST int32 subset table [nsub][stride], row i = (count_i, count_i bins),
+ *   1 <= count_i <= stride - 1 (the (Nsub, Nprj) table osemone builds,
+ *   prj/mmrrec.py:286-294);
+ * sens: device float [nsub][320][320][127] (back projections of a n, not
+ *   convolved); mask: device uint8 image; krnl: HOST float [3][17] or NULL.
+ * Temporaries (rho, a n, one work sinogram and image, two more images with
+ * resolution modelling) are allocated and freed stream-ordered on `stream`. */
+int nipet_osem_iter(const NipetGeom *g, float *img, const uint16_t *prompts, const float *r,
+                    const float *s, const float *n, const float *a, const int32_t *subs,
+                    int nsub, int stride, const float *sens, const uint8_t *mask,
+                    const float *krnl, void *stream);
+
 /* Resolution modelling: separable 17-tap convolution of nb replicate-innermost
  * device images [voxel][nb] with the host kernels krnl[3][17] (rows: z, x, y
  * taps as psf_config builds them, prj/mmrrec.py:96-155), zero outside the
@@ -191,9 +213,13 @@ typedef struct NipetLm NipetLm;
 typedef struct {
   int toff;              /* first time tag of the acquisition [ms] (lminfo) */
   int nitag;             /* seconds covered: ((last tag - toff) + 999) / 1000 */
-  int tstart, tstop;     /* frame [tstart, tstop) in seconds */
+  int tstart, tstop;     /* frame [tstart, tstop) in seconds, 0 <= tstart <= tstop <= nitag */
   int span;              /* 11 (837 sinograms), 1 (4084) or 0 (SSRB, 127) */
-  int btp;               /* 0: no bootstrap; 2: parametric (Poisson) bootstrap */
+  int btp;               /* 0: no bootstrap; 1: non-parametric (each event redrawn
+                            uniformly from the events of its patch, the 8192-word
+                            block of the session's stream that holds it; every chunk
+                            but the last must then hold a multiple of 8192 words);
+                            2: parametric (Poisson weights), lm/src/hst.cu:110-139 */
   double btprt;          /* Poisson mean of the bootstrap weights (Cnt['BTPRT']) */
   uint64_t seed;         /* Philox4x32-10 key */
   int nrep;              /* replicates 1..64; nrep > 1 stores [bin][nrep] */
@@ -205,15 +231,18 @@ typedef struct {
   const int16_t *s2cF;      /* host [86688][2] (txLUT['s2cF']) */
 } NipetLmConfig;
 
-/* Device outputs, zeroed by the caller, accumulated by every chunk. */
+/* Device outputs, zeroed by the caller, accumulated by every chunk. Only
+ * psino, dsino, phc and dhc hold every replicate; ssr, zR, zM, pvs and fan
+ * are those of the first stored replicate (rep0): with nrep > 1, randoms from
+ * `fan` are replicate rep0's estimate for every replicate. */
 typedef struct {
   uint32_t *psino, *dsino;  /* prompts / delayeds [nbins][nrep] (nbins = nsinos*86688) */
-  uint32_t *ssr;            /* SSRB prompts [127][252][344] (replicate 0), by nipet_lm_finish */
+  uint32_t *ssr;            /* SSRB prompts [127][252][344] (replicate rep0), by nipet_lm_finish */
   uint32_t *phc, *dhc;      /* head curves [nitag][nrep] */
-  uint64_t *zR, *zM;        /* centre-of-mass sums [nitag] (replicate 0) */
-  uint32_t *pvs;            /* projection views [(min(nitag,5400)+3)/4][127][344] */
-  uint32_t *fan;            /* delayed fan-sums [64][504] */
-  uint32_t *bck;            /* singles buckets [2][nitag][224] */
+  uint64_t *zR, *zM;        /* centre-of-mass sums [nitag] (replicate rep0) */
+  uint32_t *pvs;            /* projection views [(min(nitag,5400)+3)/4][127][344] (replicate rep0) */
+  uint32_t *fan;            /* delayed fan-sums [64][504] (replicate rep0) */
+  uint32_t *bck;            /* singles buckets [2][nitag][224] (not resampled) */
 } NipetLmOut;
 
 int nipet_lm_create(const NipetLmConfig *cfg, NipetLm **out);

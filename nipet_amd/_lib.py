@@ -86,6 +86,7 @@ def lib():
             'nipet_tune_get': ([I, P], I),
             'nipet_osem_subset': ([P, P, P, P, P, P, P, I, P, P, P], I),
             'nipet_osem_subset_sens': ([P, P, P, P, P, P, P, P, I, P, P, P, P], I),
+            'nipet_osem_iter': ([P, P, P, P, P, P, P, P, I, I, P, P, P, P], I),
             'nipet_fprj_b': ([P, P, P, P, I, I, I, P], I),
             'nipet_bprj_b': ([P, P, P, P, I, I, P], I),
             'nipet_osem_subset_b': ([P, P, P, P, I, P, I, P, P, I, I, P, P, P], I),

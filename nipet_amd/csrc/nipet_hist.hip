@@ -6,9 +6,18 @@
 //    word >> 29 == -4: time tag (ms in bits 0-28); word >> 29 == -3: singles;
 //  - itag = (ms - toff) / 1000 of the most recent preceding time tag (0 before
 //    the first tag); events / singles count for tstart <= itag < tstop;
-//  - bootstrap (btp = 2): weight N ~ Poisson via inversion of a uint32 CDF
-//    against Philox4x32-10(counter = (word index lo, hi, replicate, 0),
-//    key = (seed lo, hi)) output word 0; counted when 0 < N < 32.
+//  - parametric bootstrap (btp = 2): weight N ~ Poisson via inversion of a
+//    uint32 CDF against Philox4x32-10(counter = (word index lo, hi,
+//    replicate, 0), key = (seed lo, hi)) output word 0; counted when
+//    0 < N < 32;
+//  - non-parametric bootstrap (btp = 1, lm/src/hst.cu:110-126): every event
+//    word is replaced by an event word drawn uniformly from its patch, the
+//    block of LM_BLOCK words (global word index / LM_BLOCK) that holds it:
+//    draw j = floor(u nev / 2^32) with u = word 0 of Philox4x32-10(counter =
+//    (word index lo, hi, replicate, 1)), nev the patch's events, and take the
+//    j-th event of the patch in word order. The event keeps the time tag of
+//    its own position (as the reference); the drawn word gives the bin and
+//    the prompt / delayed bit.
 // Layout: with nrep replicates, sinograms and head curves are replicate-
 // innermost ([bin][nrep], [itag][nrep]): one event updates nrep consecutive
 // counters with one coalesced wave atomic. Integer atomics make every count
@@ -140,9 +149,14 @@ __device__ inline uint32_t poisson_n(uint32_t u, const uint32_t *cdf) {
 
 // REP = false: one event per lane (nrep == 1).
 // REP = true: the wave walks its events one by one, lane r = replicate r.
-template <bool REP>
+// NP: non-parametric bootstrap (btp = 1): the block's event words are
+// compacted in word order into LDS (the resampling patch) and every counted
+// event is replaced by a draw from them.
+template <bool REP, bool NP>
 __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
   __shared__ int32_t lut[NSN1];
+  __shared__ int32_t evw[NP ? LM_BLOCK : 1];
+  __shared__ int32_t wave_nev[LM_WAVES];
   __shared__ uint32_t cdf[LM_NCDF];
   __shared__ uint32_t hcp[LM_HW * (REP ? LM_MAXREP : 1)];
   __shared__ uint32_t hcd[LM_HW * (REP ? LM_MAXREP : 1)];
@@ -179,10 +193,33 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
     if (m) wl = __builtin_amdgcn_readlane(wd[j] & TMSK, 63 - __clzll(m));
   }
   if (lane == 0) wave_last[wave] = wl;
+  int nev = 0;  // events of the block (NP)
+  if constexpr (NP) {
+    int cnt = 0;
+#pragma unroll
+    for (int j = 0; j < LM_PER_LANE; j++) cnt += __popcll(__ballot(wd[j] > 0));
+    if (lane == 0) wave_nev[wave] = cnt;
+  }
   __syncthreads();
   int32_t c = carry_blk;
   for (int k = 0; k < wave; k++)
     if (wave_last[k] >= 0) c = wave_last[k];
+  if constexpr (NP) {
+    // compact the event words in word order (wave-major, then j, then lane)
+    int run = 0;
+    for (int k = 0; k < LM_WAVES; k++) {
+      if (k < wave) run += wave_nev[k];
+      nev += wave_nev[k];
+    }
+    const unsigned long long lt = (1ull << lane) - 1;
+#pragma unroll
+    for (int j = 0; j < LM_PER_LANE; j++) {
+      unsigned long long m = __ballot(wd[j] > 0);
+      if (wd[j] > 0) evw[run + __popcll(m & lt)] = wd[j];
+      run += __popcll(m);
+    }
+    __syncthreads();
+  }
 
   const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1);
   for (int j = 0; j < LM_PER_LANE; j++) {
@@ -201,9 +238,8 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
     const int rel = ms < 0 ? 0 : ms - p.toff;
     const int itag = rel / ITIME;
     const bool in_frame = itag >= p.tstart && itag < p.tstop;
-    const int32_t val = word & 0x3fffffff;
     bool ev = word > 0 && in_frame;
-    if (ev && val >= NSN1 * NSBINANG) {
+    if (!NP && ev && (word & 0x3fffffff) >= NSN1 * NSBINANG) {
       atomicAdd(p.bad, 1u);
       ev = false;
     }
@@ -219,13 +255,24 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
     if (!REP) {
       if (!ev) continue;
       uint32_t N = 1;
-      if (p.btp == 2) {
+      int32_t wv = word;
+      if constexpr (NP) {
+        uint64_t g = (uint64_t)(p.base + gi);
+        uint32_t cc[4] = {(uint32_t)g, (uint32_t)(g >> 32), (uint32_t)p.rep0, 1u};
+        philox4x32_10(cc, p.k0, p.k1);
+        wv = evw[(int)(((uint64_t)cc[0] * (uint32_t)nev) >> 32)];
+        if ((wv & 0x3fffffff) >= NSN1 * NSBINANG) {
+          atomicAdd(p.bad, 1u);
+          continue;
+        }
+      } else if (p.btp == 2) {
         uint64_t g = (uint64_t)(p.base + gi);
         uint32_t cc[4] = {(uint32_t)g, (uint32_t)(g >> 32), (uint32_t)p.rep0, 0u};
         philox4x32_10(cc, p.k0, p.k1);
         N = poisson_n(cc[0], cdf);
       }
       if (N == 0 || N >= 32) continue;
+      const int32_t val = wv & 0x3fffffff;
       const int si = val / NSBINANG, aw = val - si * NSBINANG;
       const int a = aw / NSBINS, w = aw - a * NSBINS;
       const int32_t lu = lut[si];
@@ -234,7 +281,7 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
                                         : (p.span == 1 ? (int64_t)val : (int64_t)sr * NSBINANG + aw);
       const int k = itag - itag0;
       const bool inwin = k >= 0 && k < LM_HW;
-      if ((word >> 30) & 1) {
+      if ((wv >> 30) & 1) {
         atomicAdd(p.psino + addr, N);
         if (inwin) atomicAdd(&hcp[k], N); else atomicAdd(p.phc + itag, N);
         if (inwin) {
@@ -253,6 +300,58 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
         short2 rr = p.rno[si], cr = p.s2cF[a + NSANGLES * w];
         atomicAdd(p.fan + NCRS * rr.x + cr.x, N);
         atomicAdd(p.fan + NCRS * rr.y + cr.y, N);
+      }
+    } else if constexpr (NP) {
+      // lane r draws replicate r's word: bins differ between lanes
+      unsigned long long evm = __ballot(ev);
+      while (evm) {
+        const int l = __ffsll((long long)evm) - 1;
+        evm &= evm - 1;
+        const int it = __builtin_amdgcn_readlane(itag, l);
+        const uint64_t g = (uint64_t)(p.base + w0 + j * 64 + l);
+        bool ok = false;
+        int32_t wv = 0;
+        if (lane < nr) {
+          uint32_t cc[4] = {(uint32_t)g, (uint32_t)(g >> 32), (uint32_t)(p.rep0 + lane), 1u};
+          philox4x32_10(cc, p.k0, p.k1);
+          wv = evw[(int)(((uint64_t)cc[0] * (uint32_t)nev) >> 32)];
+          ok = (wv & 0x3fffffff) < NSN1 * NSBINANG;
+          if (!ok && lane == 0) atomicAdd(p.bad, 1u);
+        }
+        if (ok) {
+          const int32_t v = wv & 0x3fffffff;
+          const int si = v / NSBINANG, aw = v - si * NSBINANG;
+          const int a = aw / NSBINS, w = aw - a * NSBINS;
+          const int32_t lu = lut[si];
+          const int s11 = lu >> 16, sr = lu & 0xffff;
+          const int64_t addr = p.span == 11 ? (int64_t)s11 * NSBINANG + aw
+                                            : (p.span == 1 ? (int64_t)v : (int64_t)sr * NSBINANG + aw);
+          const int k = it - itag0;
+          const bool inwin = k >= 0 && k < LM_HW;
+          const bool prompt = (wv >> 30) & 1;
+          atomicAdd((prompt ? p.psino : p.dsino) + addr * nr + lane, 1u);
+          uint32_t *hl = prompt ? hcp : hcd;
+          uint32_t *hg = prompt ? p.phc : p.dhc;
+          if (inwin) atomicAdd(&hl[k * nr + lane], 1u); else atomicAdd(hg + (int64_t)it * nr + lane, 1u);
+          if (lane == 0) {
+            if (prompt) {
+              if (inwin) {
+                atomicAdd(&mzr[k], (unsigned long long)sr);
+                atomicAdd(&mzm[k], 1ull);
+              } else {
+                atomicAdd(p.zR + it, (unsigned long long)sr);
+                atomicAdd(p.zM + it, 1ull);
+              }
+              if ((a == 0 || a == 126) && it < MXNITAG)
+                atomicAdd(p.pvs + (int64_t)(it >> VTIME) * SEG0 * NSBINS + sr * NSBINS + w,
+                          1u << (8 * (a == 126)));
+            } else {
+              short2 rr = p.rno[si], cr = p.s2cF[a + NSANGLES * w];
+              atomicAdd(p.fan + NCRS * rr.x + cr.x, 1u);
+              atomicAdd(p.fan + NCRS * rr.y + cr.y, 1u);
+            }
+          }
+        }
       }
     } else {
       unsigned long long evm = __ballot(ev);
@@ -506,12 +605,12 @@ int nipet_lm_create(const NipetLmConfig *c, NipetLm **out) {
     set_error("nipet_lm_create: NULL argument");
     return NIPET_EINVAL;
   }
-  if (c->nitag <= 0 || c->tstart < 0 || c->tstop < c->tstart ||
-      (c->span != 11 && c->span != 1 && c->span != 0) || (c->btp != 0 && c->btp != 2) ||
+  if (c->nitag <= 0 || c->tstart < 0 || c->tstop < c->tstart || c->tstop > c->nitag ||
+      (c->span != 11 && c->span != 1 && c->span != 0) || c->btp < 0 || c->btp > 2 ||
       c->nrep < 1 || c->nrep > LM_MAXREP || c->rep0 < 0 || c->rep0 > (1 << 30) ||
       !(c->btprt > 0.0 && c->btprt < 20.0)) {
-    set_error("nipet_lm_create: bad configuration (nitag=%d span=%d btp=%d nrep=%d)", c->nitag,
-              c->span, c->btp, c->nrep);
+    set_error("nipet_lm_create: bad configuration (nitag=%d tstart=%d tstop=%d span=%d btp=%d "
+              "nrep=%d)", c->nitag, c->tstart, c->tstop, c->span, c->btp, c->nrep);
     return NIPET_EINVAL;
   }
   NipetLm *h = new NipetLm();
@@ -565,6 +664,14 @@ int nipet_lm_chunk(NipetLm *h, const int32_t *words, int64_t n, const NipetLmOut
   }
   if (n == 0) return NIPET_OK;
   const NipetLmConfig &c = h->cfg;
+  if (c.btp == 1 && h->base % LM_BLOCK != 0) {
+    // the resampling patches are the LM_BLOCK-word blocks of the session's
+    // word stream: a chunk must start on a patch boundary
+    set_error("nipet_lm_chunk: non-parametric bootstrap needs every chunk but the last to hold "
+              "a multiple of %d words (this chunk starts at word %lld)", LM_BLOCK,
+              (long long)h->base);
+    return NIPET_EINVAL;
+  }
   hipStream_t st = (hipStream_t)stream;
   int64_t nblk = (n + LM_BLOCK - 1) / LM_BLOCK;
   if (nblk > INT32_MAX) {
@@ -615,10 +722,16 @@ int nipet_lm_chunk(NipetLm *h, const int32_t *words, int64_t n, const NipetLmOut
   p.sgl_count = h->sgl_count;
   p.sgl_cap = h->sgl_cap;
   p.bad = h->bad;
-  if (c.nrep == 1)
-    k_lm_hist<false><<<(unsigned)nblk, LM_THREADS, 0, st>>>(p);
-  else
-    k_lm_hist<true><<<(unsigned)nblk, LM_THREADS, 0, st>>>(p);
+  if (c.btp == 1) {
+    if (c.nrep == 1)
+      k_lm_hist<false, true><<<(unsigned)nblk, LM_THREADS, 0, st>>>(p);
+    else
+      k_lm_hist<true, true><<<(unsigned)nblk, LM_THREADS, 0, st>>>(p);
+  } else if (c.nrep == 1) {
+    k_lm_hist<false, false><<<(unsigned)nblk, LM_THREADS, 0, st>>>(p);
+  } else {
+    k_lm_hist<true, false><<<(unsigned)nblk, LM_THREADS, 0, st>>>(p);
+  }
   k_lm_sgl_first<<<256, 256, 0, st>>>(h->sgl_idx, h->sgl_word, h->sgl_ms, h->sgl_count,
                                       h->sgl_cap, c.nitag, h->first_nz);
   k_lm_sgl_apply<<<256, 256, 0, st>>>(h->sgl_idx, h->sgl_word, h->sgl_ms, h->sgl_count,

@@ -1394,4 +1394,83 @@ int nipet_osem_subset_sens(const NipetGeom *g, float *img, const uint16_t *promp
   return NIPET_OK;
 }
 
+int nipet_osem_iter(const NipetGeom *g, float *img, const uint16_t *prompts, const float *r,
+                    const float *s, const float *n, const float *a, const int32_t *subs,
+                    int nsub, int stride, const float *sens, const uint8_t *mask,
+                    const float *krnl, void *stream) {
+  int rc = check_geom(g, "nipet_osem_iter");
+  if (rc) return rc;
+  if (!img || !prompts || !n || !a || !subs || !sens || !mask || nsub < 1 || stride < 2) {
+    set_error("nipet_osem_iter: NULL buffer, nsub < 1 or stride < 2");
+    return NIPET_EINVAL;
+  }
+  int maxc = 0;
+  for (int i = 0; i < nsub; i++) {
+    const int c = subs[(int64_t)i * stride];
+    if (c < 1 || c > stride - 1 || c > g->naw) {
+      set_error("nipet_osem_iter: subset %d holds %d bins (must be 1 .. min(stride - 1 = %d, "
+                "naw = %d))", i, c, stride - 1, g->naw);
+      return NIPET_EINVAL;
+    }
+    maxc = c > maxc ? c : maxc;
+  }
+  const bool rm = krnl && krnl[0] >= 0.f;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t nel = (int64_t)g->naw * g->nsn;
+  // per-call temporaries, stream ordered (the reference allocates and frees
+  // its device copies in every call, prj/src/recon.cu:333-419, 511-520)
+  float *rho = nullptr, *an = nullptr, *wsino = nullptr, *wimg = nullptr, *wimg2 = nullptr,
+        *srm = nullptr;
+  int32_t *dsubs = nullptr;
+  auto release = [&]() {
+    void *ps[] = {rho, an, wsino, wimg, wimg2, srm, dsubs};
+    for (void *q : ps)
+      if (q) (void)hipFreeAsync(q, st);
+  };
+  auto get = [&](void **p, size_t bytes) -> int {
+    hipError_t e = hipMallocAsync(p, bytes, st);
+    if (e != hipSuccess) {
+      *p = nullptr;
+      release();
+      return hip_fail(e, "nipet_osem_iter: hipMallocAsync");
+    }
+    return NIPET_OK;
+  };
+  if ((rc = get((void **)&rho, nel * sizeof(float))) || (rc = get((void **)&an, nel * sizeof(float))) ||
+      (rc = get((void **)&wsino, (size_t)maxc * g->nsn * sizeof(float))) ||
+      (rc = get((void **)&wimg, NVOX * sizeof(float))) ||
+      (rc = get((void **)&dsubs, (size_t)nsub * stride * sizeof(int32_t))))
+    return rc;
+  if (rm && ((rc = get((void **)&wimg2, NVOX * sizeof(float))) ||
+             (rc = get((void **)&srm, NVOX * sizeof(float)))))
+    return rc;
+  hipError_t e = hipMemcpyAsync(dsubs, subs, (size_t)nsub * stride * sizeof(int32_t),
+                                hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) {
+    release();
+    return hip_fail(e, "nipet_osem_iter: subset table upload");
+  }
+  // rho = (r + s) / (a n) once per call (prj/src/recon.cu:381-398)
+  k_osem_corr<<<ew_blocks(nel), 256, 0, st>>>(rho, an, a, n, r, s, nel);
+  for (int i = 0; i < nsub && rc == NIPET_OK; i++) {
+    const int c = subs[(int64_t)i * stride];
+    const int32_t *rows = dsubs + (int64_t)i * stride + 1;
+    const float *si = sens + (int64_t)i * NVOX;
+    if (rm) {
+      // the reference convolves the sensitivity images in every call
+      // (prj/src/recon.cu:441-445); here one at a time, before its subset
+      if ((rc = psf_conv(srm, si, wimg, krnl, 1, st))) break;
+      rc = nipet_osem_subset_rm_b(g, img, prompts, rho, 0, srm, 0, mask, rows, c, 1, krnl, wsino,
+                                  wimg, wimg2, stream);
+    } else {
+      rc = nipet_osem_subset_b(g, img, prompts, rho, 0, si, 0, mask, rows, c, 1, wsino, wimg,
+                               stream);
+    }
+  }
+  release();
+  if (rc) return rc;
+  NIPET_HIP(hipGetLastError());
+  return NIPET_OK;
+}
+
 }  // extern "C"

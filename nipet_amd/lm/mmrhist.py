@@ -5,8 +5,10 @@
 * ``mmrhist`` / ``hist`` -- ``lm/mmrhist.py:25-180``: histogram a frame of a
   list-mode file into span-11 prompt/delayed sinograms, SSRB sinogram, head
   curves, centre of mass, projection views, fan-sums and bucket singles, with
-  optional parametric bootstrap (``Cnt['BTP'] = 2``, Poisson mean
-  ``Cnt['BTPRT']``) from a reproducible counter-based generator.
+  optional bootstrap from a reproducible counter-based generator:
+  non-parametric (``Cnt['BTP'] = 1``, each event redrawn from the events of
+  its 8192-word patch) or parametric (``Cnt['BTP'] = 2``, Poisson weights of
+  mean ``Cnt['BTPRT']``).
 * ``LmHistogram`` -- the device session behind both: words are handed over in
   chunks (device tensors), the time-tag state carries across chunks.
 * ``randoms`` / ``rand`` -- ``lm/mmrhist.py:188-221``: randoms sinogram and
@@ -27,7 +29,7 @@ from .. import _lib
 log = logging.getLogger(__name__)
 
 __all__ = ['lminfo', 'mmrhist', 'hist', 'LmHistogram', 'hist_words', 'Randoms', 'randoms',
-           'rand']
+           'rand', 'dynamic_timings']
 
 CHUNK_WORDS = 1 << 26      # 256 MB of words per transfer
 
@@ -81,6 +83,12 @@ class LmHistogram:
         self.btprt = Cnt.get('BTPRT', 1.0) if btprt is None else btprt
         self.nitag, self.nrep = int(nitag), int(nrep)
         tstop = self.nitag if tstop is None else tstop
+        if not 0 <= tstart <= tstop <= self.nitag:
+            raise ValueError(f'frame [{tstart}, {tstop}) s outside the acquisition '
+                             f'[0, {self.nitag}) s')
+        if self.btp not in (0, 1, 2):
+            raise ValueError(f'bootstrap type {self.btp} (0: none, 1: non-parametric, '
+                             '2: parametric)')
         self.nsinos = {11: Cnt['NSN11'], 1: Cnt['NSN1'], 0: Cnt['NSEG0']}[self.span]
         self._keep = [np.ascontiguousarray(ax['sn1_sn11'], np.int16),
                       np.ascontiguousarray(ax['sn1_ssrb'], np.int16),
@@ -271,6 +279,40 @@ def mmrhist(datain, scanner_params, t0=0, t1=0, outpath='', frms=None, use_store
     Cnt, txLUT, axLUT = scanner_params['Cnt'], scanner_params['txLUT'], scanner_params['axLUT']
     return hist(datain, txLUT, axLUT, Cnt, frms=frms, use_stored=use_stored, store=store,
                 outpath=outpath, t0=t0, t1=t1, cmass_sig=cmass_sig)
+
+
+def dynamic_timings(flist, offset=0):
+    """Start and end times of dynamic frames (``lm/mmrhist.py:492-556``).
+    ``flist`` is a list of frame durations [15, 15, 30, ...] or ``['def',
+    [repetitions, duration], ...]``; ``offset`` adds a first frame [0, offset]
+    (durations list) or a zero-repetition entry (``'def'`` list), as the
+    reference does. Returns {'total': end time, 'frames': uint16 durations,
+    'timings': ['timings', [t0, t1], ...]}."""
+    from collections.abc import Collection
+    from numbers import Integral
+    if not isinstance(flist, Collection) or isinstance(flist, str):
+        raise TypeError('Wrong type of frame data input')
+    if all(isinstance(t, Integral) for t in flist):
+        tsum = offset
+        t_frames = ['timings', [0, offset]] if offset > 0 else ['timings']
+        for d in flist:
+            t_frames.append([tsum, tsum + d])
+            tsum += d
+        frms = np.uint16(flist)
+    elif flist[0] == 'def' and all(isinstance(t, Collection) and len(t) == 2
+                                   for t in flist[1:]):
+        defs = [list(t) for t in flist[1:]]
+        if offset > 0:
+            defs.insert(0, [0, offset])
+        farray = np.asarray(defs, dtype=np.int64)
+        frms = np.repeat(farray[:, 1], farray[:, 0]).astype(np.uint16)
+        t_frames, tsum = ['timings'], 0
+        for d in frms:
+            t_frames.append([tsum, tsum + int(d)])
+            tsum += int(d)
+    else:
+        raise TypeError('Unrecognised time frame definitions.')
+    return {'total': tsum, 'frames': frms, 'timings': t_frames}
 
 
 # ------------------------------------------------------------------------------

@@ -28,7 +28,14 @@ from .. import _lib
 from .osem import OsemBatch, osem_iteration_batch, osem_prepare_batch
 
 __all__ = ['GROUP', 'autotune', 'prepare_groups', 'OsemGraph', 'bootstrap_prompts',
-           'frame_prompts', 'bootstrap_osem', 'frames_osem']
+           'frame_prompts', 'bootstrap_osem', 'frames_osem', 'FRAMES24', 'replicates_and_frames']
+
+# 24 dynamic frames of uneven length (14 x 10 s, 4 x 15 s, 5 x 20 s, 1 x 300 s)
+# covering a 600 s acquisition
+FRAMES24 = ([(10 * i, 10 * i + 10) for i in range(14)] +
+            [(140 + 15 * i, 155 + 15 * i) for i in range(4)] +
+            [(200 + 20 * i, 220 + 20 * i) for i in range(5)] +
+            [(300, 600)])
 
 GROUP = 4  # replicates per contiguous group (the widest kernel instance)
 
@@ -97,7 +104,7 @@ def _per_rep(x):
 
 
 def prepare_groups(scanner_params, prompt_groups, asng, nsng, rsng=None, ssng=None, sens=None,
-                   psf=None):
+                   psf=None, mask_radius=29.):
     """One ``OsemBatch`` per replicate group (``prompt_groups[i]``: CUDA uint16
     (Naw, 837, g_i), the first group the largest). Attenuation and norm are
     shared; randoms / scatter are shared (Naw, 837) or per replicate
@@ -116,7 +123,7 @@ def prepare_groups(scanner_params, prompt_groups, asng, nsng, rsng=None, ssng=No
             def sl(x):
                 return x[..., off:off + g].contiguous() if _per_rep(x) else x
             st = osem_prepare_batch(
-                scanner_params, p, asng, nsng, sl(rsng), sl(ssng),
+                scanner_params, p, asng, nsng, sl(rsng), sl(ssng), mask_radius=mask_radius,
                 sens=sens if first is None else first.sens, work=work, psf=psf)
         if first is None:
             first = st
@@ -157,7 +164,10 @@ class OsemGraph:
                     osem_iteration_batch(st, img)
         torch.cuda.synchronize()
 
-    def run(self, img0=None):
+    def run_groups(self, img0=None):
+        """Reset the images (ones, or ``img0`` [320, 320, 127, nb]), launch the
+        graph and return the per-group images [320, 320, 127, g] (the graph's
+        own buffers: valid until the next run)."""
         off = 0
         for img in self.imgs:
             if img0 is None:
@@ -166,7 +176,12 @@ class OsemGraph:
                 img.copy_(img0[..., off:off + img.shape[3]])
             off += img.shape[3]
         self.graph.replay()
-        return self.imgs[0] if len(self.imgs) == 1 else _torch().cat(self.imgs, dim=3)
+        return self.imgs
+
+    def run(self, img0=None):
+        """``run_groups`` joined into one [320, 320, 127, nb] tensor."""
+        imgs = self.run_groups(img0)
+        return imgs[0] if len(imgs) == 1 else _torch().cat(imgs, dim=3)
 
 
 def bootstrap_prompts(words, scanner_params, nrep, seed, rep0=0, t0=0, t1=0, hist=None,
@@ -288,3 +303,63 @@ def frames_osem(words, scanner_params, frames, asng, nsng, rho_frames, itr=4, tu
                 osem_iteration_batch(x, im)
         img = torch.cat(imgs, dim=3)
     return img, info
+
+
+def replicates_and_frames(words, scanner_params, asng, nsng, rs, nrep=64, seed=1,
+                          frames=FRAMES24, itr=4, tune=True, psf=None):
+    """Bootstrap replicates 0 .. nrep-1 of a whole acquisition and its dynamic
+    frames [(t0, t1), ...], reconstructed ``itr`` x 14 as ONE batch: one
+    list-mode pass for all replicates, one session per frame, shared
+    attenuation / norm (Naw, 837) and sensitivity images, randoms + scatter
+    ``rs`` (Naw, 837) of the whole acquisition for every replicate and
+    rs * (t1 - t0) / duration per frame, and every subset update of every
+    replicate and frame group captured in one HIP graph. Returns (list of
+    nrep replicate images, list of frame images) as [320, 320, 127] views of
+    the graph's buffers, and an info dict (autotune, timing)."""
+    import time
+    torch = _torch()
+    from ..lm.mmrhist import tag_range
+    tm = {}
+
+    def mark(name, t0=[None]):
+        torch.cuda.synchronize()
+        now = time.perf_counter()
+        if t0[0] is not None:
+            tm[name] = round(now - t0[0], 4)
+        t0[0] = now
+
+    mark(None)
+    rep_groups, h = bootstrap_prompts(words, scanner_params, nrep, seed, group=GROUP)
+    del h
+    mark('histogram_s')
+    tf, tl = tag_range(words)
+    dur = ((tl - tf) + 999) // 1000
+    fprompts, dtot = frame_prompts(words, scanner_params, frames)
+    nf = fprompts.shape[2]
+    frho = torch.stack([rs * ((t1 - t0) / dur) for t0, t1 in frames], dim=2).contiguous()
+    mark('frame_histograms_s')
+    reps = prepare_groups(scanner_params, rep_groups, asng, nsng, rs, None, psf=psf)
+    fgroups = [fprompts[..., f0:f0 + GROUP].contiguous() for f0 in range(0, nf, GROUP)]
+    del fprompts
+    first = reps[0]
+    frm = []
+    off = 0
+    for p in fgroups:
+        g = p.shape[2]
+        frm.append(osem_prepare_batch(scanner_params, p, asng, nsng,
+                                      frho[..., off:off + g].contiguous(), None,
+                                      sens=first.sens, psf=psf,
+                                      work=(first.work_sino, first.work_img, first.work_img2)))
+        off += g
+    del frho
+    mark('prepare_s')
+    info = {'tune': autotune(first) if tune else None, 'delayed': dtot}
+    mark('tune_s')
+    graph = OsemGraph(reps + frm, itr)
+    mark('capture_s')
+    imgs = graph.run_groups()
+    mark('osem_s')
+    out = [img[..., b] for img in imgs for b in range(img.shape[3])]
+    info['timing'] = tm
+    info['graph'] = graph
+    return out[:nrep], out[nrep:], info

@@ -20,7 +20,7 @@ import numpy as np
 log = logging.getLogger(__name__)
 
 __all__ = ['get_subsets14', 'psf_config', 'fwhm2sig', 'osemone', 'osem_prepare', 'osem_run',
-           'RI_LUT', 'quant_factor']
+           'RI_LUT', 'quant_factor', 'recon_inputs', 'rec_output']
 
 # Radioisotopes: half-life [s] and positron branching fraction. The reference
 # reads them from ``resources.riLUT`` of its un-vendored ``niftypet.ninst``
@@ -135,6 +135,105 @@ def osem_run(*args, **kwargs):
     return f(*args, **kwargs)
 
 
+def recon_inputs(datain, mumaps, hst, scanner_params, recmod=3, attnsino=None, sctsino=None,
+                 randsino=None, normcomp=None, normsino=None):
+    """The sinograms ``osemone`` reconstructs from (``prj/mmrrec.py:196-265``),
+    in the device layout (Naw, nsn): dict with ``psng`` (prompts of ``hst``),
+    ``asng`` (attenuation factors: ones for recmod 0, ``attnsino``, or the
+    attenuation projection of the mu-maps, kept on the device), ``nsng``
+    (``normsino``, else from ``normcomp`` or the component files of
+    ``datain`` with ``hst['buckets']``, else ones), ``rsng`` (``randsino``,
+    else from ``hst['fansums']``, else None), ``ssng`` (``sctsino`` for
+    recmod >= 2, else None) and ``ncmp`` (the normalisation components used)."""
+    from ..img import mmrimg
+    from . import mmrprj
+    Cnt, txLUT = scanner_params['Cnt'], scanner_params['txLUT']
+
+    def dev(s):
+        if s is None:
+            return None
+        if hasattr(s, 'is_cuda'):
+            if s.dim() == 3:
+                from ..mmraux import remgaps
+                return remgaps(s, txLUT, Cnt)
+            return s
+        s = np.asarray(s)
+        if s.ndim == 3:
+            from ..mmraux import remgaps
+            return remgaps(s, txLUT, Cnt)
+        return s
+
+    psng = dev(hst['psino'])
+    if recmod == 0:
+        asng = np.ones(tuple(psng.shape), dtype=np.float32)
+    elif attnsino is not None:
+        asng = dev(attnsino)
+    else:
+        import torch
+        muh, muo = mumaps
+        mus = torch.from_numpy(np.ascontiguousarray(
+            mmrimg.convert2dev(np.asarray(muo) + np.asarray(muh), Cnt), np.float32)).cuda()
+        asng = mmrprj.frwd_prj(mus, scanner_params, dev_out=True, attenuation=True)
+    ncmp = normcomp
+    if ncmp is None and datain.get('nrm_bf') and (datain.get('nrm_ima') or datain.get('nrm_dcm')):
+        from ..mmrnorm import get_components
+        ncmp = get_components(datain, Cnt)[0]
+    if normsino is not None:
+        nsng = dev(normsino)
+    elif ncmp is not None:
+        # prj/mmrrec.py:209-215: the normalisation sinogram from its components
+        from ..mmrnorm import NormSino
+        nsng = NormSino(scanner_params)(ncmp, hst['buckets'])
+    else:
+        nsng = np.ones(tuple(psng.shape), np.float32)
+    if randsino is not None:
+        rsng = dev(randsino)
+    elif hst.get('fansums') is not None:
+        from ..lm.mmrhist import Randoms
+        rsng, _ = Randoms(scanner_params)(hst['fansums'])
+    else:
+        rsng = None
+    ssng = None if (sctsino is None or recmod < 2) else dev(sctsino)
+    return {'psng': psng, 'asng': asng, 'nsng': nsng, 'rsng': rsng, 'ssng': ssng, 'ncmp': ncmp}
+
+
+def rec_output(img, iterations, datain, hst, scanner_params, ncmp=None, decay_ref_time=None,
+               fwhm=0., frmno='', fcomment='', outpath=None, fout=None):
+    """``osemone``'s result from the device image ``img`` (320, 320, 127) after
+    ``iterations`` iterations (``prj/mmrrec.py:384-470``): quantification and
+    decay correction, E7 image, the reference's output file name and, with
+    ``fwhm > 0``, the smoothed image. Returns ``RecOut(im, fpet, imsmo, fsmo,
+    affine)``."""
+    from collections import namedtuple
+
+    from ..img import mmrimg
+    Cnt = scanner_params['Cnt']
+    qscale = quant_factor(Cnt, hst, ncmp, decay_ref_time)
+    if qscale != 1.0:
+        log.info('applying decay correction and quantification factor %r', qscale)
+        img = img * qscale
+    im = mmrimg.convert2e7(img, Cnt)
+    # output file name of the reference (prj/mmrrec.py:175-186, 406-413)
+    opth = outpath if outpath is not None else (
+        os.path.join(datain['corepath'], 'reconstructed') if datain.get('corepath') else '')
+    fpet = None
+    if fout is not None:
+        fpet = os.path.join(opth, os.path.basename(fout).split('.')[0] +
+                            f'_itr{iterations}{fcomment}.nii.gz')
+    elif datain.get('lm_bf') and 't0' in hst and 't1' in hst:
+        fpet = os.path.join(opth, os.path.basename(datain['lm_bf']).split('.')[0] +
+                            f"{frmno}_t{hst['t0']}-{hst['t1']}sec_itr{iterations}"
+                            f"{fcomment}.nii.gz")
+    im_smo = None
+    if fwhm > 0:
+        # prj/mmrrec.py:420-421
+        from scipy import ndimage as ndi
+        im_smo = ndi.gaussian_filter(im, fwhm2sig(fwhm, voxsize=Cnt['SZ_VOXY'] * 10),
+                                     mode='mirror')
+    RecOut = namedtuple('RecOut', 'im, fpet, imsmo, fsmo, affine')
+    return RecOut(im, fpet, im_smo, None, None)
+
+
 def osemone(datain, mumaps, hst, scanner_params, recmod=3, itr=4, fwhm=0., psf=None,
             mask_radius=29., decay_ref_time=None, attnsino=None, sctsino=None, randsino=None,
             normcomp=None, emmskS=False, frmno='', fcomment='', outpath=None, fout=None,
@@ -149,7 +248,9 @@ def osemone(datain, mumaps, hst, scanner_params, recmod=3, itr=4, fwhm=0., psf=N
     ``datain`` with ``hst['buckets']``, as ``prj/mmrrec.py:209-215``, else
     ones), ``randsino`` (additive; estimated from ``hst['fansums']``
     when None, as ``prj/mmrrec.py:244-250``, zero without fan-sums) and
-    ``sctsino`` (additive; zero when None).
+    ``sctsino`` (additive; zero when None). ``emmskS`` only shapes the
+    reference's scatter estimate (``prj/mmrrec.py:365``), which is not modelled
+    here, so it has no effect.
     As the reference, the image is scaled by ``quant_factor`` (decay correction
     and quantification, when the normalisation components are known and ``hst``
     carries ``t0`` / ``dur``) and, with ``fwhm > 0``, also returned smoothed by
@@ -160,82 +261,20 @@ def osemone(datain, mumaps, hst, scanner_params, recmod=3, itr=4, fwhm=0., psf=N
     affine (which needs the bed position from the DICOM headers). Writing
     NIfTI files (``store_img``, ``store_itr``) is not part of this package.
     """
-    from collections import namedtuple
-
-    from ..img import mmrimg
     from . import osem as _osem
-    from . import mmrprj
-    Cnt, txLUT = scanner_params['Cnt'], scanner_params['txLUT']
     if store_img or store_itr:
         raise NotImplementedError('osemone: writing NIfTI images (store_img / store_itr) is not '
                                   'part of nipet_amd; save RecOut.im with a NIfTI library')
-
-    def dev(s):
-        if s is None:
-            return None
-        s = np.asarray(s)
-        if s.ndim == 3:
-            from ..mmraux import remgaps
-            return remgaps(s, txLUT, Cnt)
-        return s
-
-    psng = dev(hst['psino'])
-    if recmod == 0:
-        asng = np.ones(psng.shape, dtype=np.float32)
-    elif attnsino is not None:
-        asng = dev(attnsino).astype(np.float32)
-    else:
-        muh, muo = mumaps
-        mus = mmrimg.convert2dev(np.asarray(muo) + np.asarray(muh), Cnt)
-        asng = mmrprj.frwd_prj(mus, scanner_params, dev_out=True, attenuation=True)
-        asng = np.asarray(asng.cpu() if hasattr(asng, 'cpu') else asng)
-    ncmp = normcomp
-    if ncmp is None and datain.get('nrm_bf') and (datain.get('nrm_ima') or datain.get('nrm_dcm')):
-        from ..mmrnorm import get_components
-        ncmp = get_components(datain, Cnt)[0]
-    if normsino is not None:
-        nsng = dev(normsino)
-    elif ncmp is not None:
-        # prj/mmrrec.py:209-215: the normalisation sinogram from its components
-        from ..mmrnorm import NormSino
-        nsng = NormSino(scanner_params)(ncmp, hst['buckets'])
-    else:
-        nsng = np.ones(psng.shape, np.float32)
-    if randsino is not None:
-        rsng = dev(randsino)
-    elif hst.get('fansums') is not None:
-        from ..lm.mmrhist import Randoms
-        rsng, _ = Randoms(scanner_params)(hst['fansums'])
-    else:
-        rsng = None
-    ssng = None if (sctsino is None or recmod < 2) else dev(sctsino)
-    psfkernel = psf_config(psf, Cnt)
+    if ret_sinos:
+        raise NotImplementedError('osemone: ret_sinos returns the scatter model\'s sinograms; '
+                                  'scatter is not modelled here (pass it as sctsino)')
+    d = recon_inputs(datain, mumaps, hst, scanner_params, recmod, attnsino, sctsino, randsino,
+                     normcomp, normsino)
+    psfkernel = psf_config(psf, scanner_params['Cnt'])
     stime = time.time()
-    state = _osem.osem_prepare(scanner_params, psng, asng, nsng, rsng, ssng,
-                               mask_radius=mask_radius, psf=psfkernel)
+    state = _osem.osem_prepare(scanner_params, d['psng'], d['asng'], d['nsng'], d['rsng'],
+                               d['ssng'], mask_radius=mask_radius, psf=psfkernel)
     img = _osem.osem_run(state, itr)
     log.info('recon time: %.3g', time.time() - stime)
-    qscale = quant_factor(Cnt, hst, ncmp, decay_ref_time)
-    if qscale != 1.0:
-        log.info('applying decay correction and quantification factor %r', qscale)
-        img *= qscale
-    im = mmrimg.convert2e7(img, Cnt)
-    # output file name of the reference (prj/mmrrec.py:175-186, 406-413)
-    opth = outpath if outpath is not None else (
-        os.path.join(datain['corepath'], 'reconstructed') if datain.get('corepath') else '')
-    fpet = None
-    if fout is not None:
-        fpet = os.path.join(opth, os.path.basename(fout).split('.')[0] +
-                            f'_itr{state.iterations}{fcomment}.nii.gz')
-    elif datain.get('lm_bf') and 't0' in hst and 't1' in hst:
-        fpet = os.path.join(opth, os.path.basename(datain['lm_bf']).split('.')[0] +
-                            f"{frmno}_t{hst['t0']}-{hst['t1']}sec_itr{state.iterations}"
-                            f"{fcomment}.nii.gz")
-    im_smo = None
-    if fwhm > 0:
-        # prj/mmrrec.py:420-421
-        from scipy import ndimage as ndi
-        im_smo = ndi.gaussian_filter(im, fwhm2sig(fwhm, voxsize=Cnt['SZ_VOXY'] * 10),
-                                     mode='mirror')
-    RecOut = namedtuple('RecOut', 'im, fpet, imsmo, fsmo, affine')
-    return RecOut(im, fpet, im_smo, None, None)
+    return rec_output(img, state.iterations, datain, hst, scanner_params, d['ncmp'],
+                      decay_ref_time, fwhm, frmno, fcomment, outpath, fout)

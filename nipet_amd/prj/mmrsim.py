@@ -10,13 +10,16 @@
   Poisson(scale (a n FP(phantom)) + randoms + scatter) with flat randoms
   (12 % of the trues), smooth scatter (25 % of the trues), norm U(0.8, 1.2),
   scaled to a given number of expected counts. Generated on the GPU.
+* ``lm_workload`` -- a list-mode acquisition of the same data model: prompt
+  bins drawn from the expected prompts of ``osem_inputs`` (trues + randoms +
+  scatter), with the randoms + scatter sinogram at the stream's count level.
 """
 import numpy as np
 
 VOX, VOXZ = 0.208626, 0.203125
 AXES = (9.0, 7.0, 8.0)
 
-__all__ = ['simulate_sino', 'ellipsoid_phantom', 'mumap_phantom', 'osem_inputs']
+__all__ = ['simulate_sino', 'ellipsoid_phantom', 'mumap_phantom', 'osem_inputs', 'lm_workload']
 
 
 def _grid_axes():
@@ -137,3 +140,26 @@ def osem_inputs(scanner_params, counts=3e8, seed=4, img=None, mu=None):
     prompts = torch.poisson(lam, generator=g).clamp_(0, 65535).to(torch.int32).to(torch.uint16)
     return {'prompts': prompts, 'attn': attn, 'norm': norm, 'randoms': scale * rnd,
             'scatter': scale * sct, 'img': img, 'mu': mu, 'scale': scale}
+
+
+def lm_workload(scanner_params, d, seconds=600, per_ms=1000, seed=5, prompt_frac=0.85):
+    """List-mode stream of ``seconds`` s (1 ms tags, ``per_ms`` events per ms,
+    ``prompt_frac`` prompts) whose prompt bins follow the expected prompts of
+    the ``osem_inputs`` data set ``d`` (a n FP(phantom) scaled, + randoms +
+    scatter). Returns (device int32 words, randoms + scatter (Naw, 837) at the
+    stream's prompt count: the additive term of a reconstruction of the whole
+    stream, or of a bootstrap replicate of it; a frame of duration T takes
+    T / ``seconds`` of it)."""
+    import torch
+
+    from ..lm.lmsim import synth_lm
+    from .mmrprj import frwd_prj
+    emis = frwd_prj(torch.from_numpy(d['img']).cuda(), scanner_params, dev_out=True)
+    rs = d['randoms'] + d['scatter']
+    lam = d['scale'] * d['attn'] * d['norm'] * emis + rs
+    del emis
+    words = synth_lm(scanner_params, seconds, per_ms, weights=lam, seed=seed,
+                     prompt_frac=prompt_frac)
+    nprompt = int(((words > 0) & (((words >> 30) & 1) == 1)).sum(dtype=torch.int64))
+    rs = rs * (nprompt / float(lam.sum(dtype=torch.float64)))
+    return words, rs.contiguous()

@@ -23,7 +23,8 @@ from ..geom import get_geom
 
 log = logging.getLogger(__name__)
 
-__all__ = ['OsemState', 'osem_prepare', 'osem_run', 'sensitivity', 'cylinder_mask']
+__all__ = ['OsemState', 'osem_prepare', 'osem_run', 'sensitivity', 'cylinder_mask',
+           'prompts_u16']
 
 
 NVOX = 320 * 320 * 127
@@ -70,6 +71,27 @@ def cylinder_mask(Cnt, radius=29.):
         m = d[:, :, None].expand(d.shape[0], d.shape[1], nvz).contiguous()
         _MASKS[key] = m
     return m
+
+
+def prompts_u16(psng):
+    """Prompt counts as uint16, saturated to 0 .. 65535 (fractional values
+    truncated), for NumPy arrays and torch tensors alike (the result stays in
+    the input's library and device). ``petprj.osem`` takes uint16 prompts
+    (``prj/src/prj_module.cu:649-651``); the reference's histogrammer packs
+    16-bit counts, which wrap at 65536 (``lm/src/hst.cu:173``), whereas the
+    histogram here counts in 32 bits and saturates on the way to uint16."""
+    torch = _torch()
+    if isinstance(psng, torch.Tensor):
+        if psng.dtype == torch.uint16:
+            return psng
+        x = psng.trunc() if psng.is_floating_point() else psng.to(torch.int64)
+        return x.clamp(0, 65535).to(torch.int32).to(torch.uint16)
+    x = np.asarray(psng)
+    if x.dtype == np.uint16:
+        return x
+    if x.dtype.kind == 'f':
+        x = np.trunc(x)
+    return np.clip(x, 0, 65535).astype(np.uint16)
 
 
 def _dev(x, dtype):
@@ -169,13 +191,9 @@ def osem_prepare(scanner_params, psng, asng, nsng, rsng=None, ssng=None, mask_ra
     Cnt = scanner_params['Cnt']
     shape = (geom.naw, geom.nsn)
     if isinstance(psng, torch.Tensor):
-        prompts = psng.to('cuda')
-        if prompts.dtype != torch.uint16:
-            prompts = prompts.to(torch.int32).clamp(0, 65535).to(torch.uint16)
-        prompts = prompts.contiguous()
+        prompts = prompts_u16(psng.to('cuda')).contiguous()
     else:
-        # the reference hands petprj.osem a uint16 array (a cast truncates)
-        prompts = _dev(np.asarray(psng).astype(np.uint16), torch.uint16)
+        prompts = _dev(prompts_u16(psng), torch.uint16)
     if tuple(prompts.shape) != shape:
         raise ValueError(f'prompts must be device-shaped {shape}')
     a = _dev(asng, torch.float32)
@@ -303,11 +321,8 @@ def osem_prepare_batch(scanner_params, prompts, asng, nsng, rsng=None, ssng=None
     from .mmrrec import get_subsets14
     geom = get_geom(scanner_params)
     Cnt = scanner_params['Cnt']
-    p = prompts if isinstance(prompts, torch.Tensor) else torch.from_numpy(np.asarray(prompts))
-    p = p.to('cuda')
-    if p.dtype != torch.uint16:
-        p = p.to(torch.int32).clamp(0, 65535).to(torch.uint16)
-    p = p.contiguous()
+    p = prompts if isinstance(prompts, torch.Tensor) else torch.from_numpy(prompts_u16(prompts))
+    p = prompts_u16(p.to('cuda')).contiguous()
     if p.dim() != 3 or tuple(p.shape[:2]) != (geom.naw, geom.nsn):
         raise ValueError(f'prompts must be (Naw, nsn, nb) = ({geom.naw}, {geom.nsn}, nb)')
     nb = p.shape[2]

@@ -21,7 +21,16 @@
  *    counted when 0 < N < 32. Here N is drawn by inversion of a uint32 CDF
  *    table against the first output word of Philox4x32-10 (Salmon, Moraes,
  *    Dror, Shaw, SC'11) with counter (word index lo, hi, replicate, 0) and key
- *    (seed lo, seed hi), replacing the reference's clock-seeded cuRAND.
+ *    (seed lo, seed hi), replacing the reference's clock-seeded cuRAND;
+ *  - non-parametric bootstrap (btp = 1, lm/src/hst.cu:110-126): an event
+ *    word is replaced by an event word drawn uniformly from its patch, the
+ *    8192-word block (global word index / 8192) that holds it: with u the
+ *    first output word of Philox4x32-10 (counter (word index lo, hi,
+ *    replicate, 1)), the floor(u nev / 2^32)-th of the patch's nev event words
+ *    in word order. The event keeps the time tag of its position; the drawn
+ *    word gives the bin and the prompt bit. (The reference draws positions of
+ *    its thread's patch with cuRAND until it hits an event: also uniform over
+ *    the patch's events.)
  * Prompt and delayed counts are kept in separate uint32 arrays (the reference
  * packs both into one uint32, lm/src/hst.cu:173,194).
  */
@@ -39,6 +48,7 @@
 #define MXNITAG 5400
 #define VTIME 2
 #define ITIME 1000
+#define LM_PATCH 8192
 
 /* Philox4x32-10 */
 static void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
@@ -96,21 +106,41 @@ int64_t orc_lm_hist(const int32_t *words, int64_t n, int64_t base, int toff, int
   int itag = 0, itagu = 0;
   int64_t counted = 0;
   uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  /* btp = 1: event words of the current patch, in word order */
+  int32_t pev[LM_PATCH];
+  int64_t patch = -1;
+  uint32_t nev = 0;
   for (int64_t i = 0; i < n; i++) {
     int32_t word = words[i];
-    int32_t val = word & 0x3fffffff;
     if (word > 0) {
       if (itag < tstart || itag >= tstop) continue;
-      int si = val / NSBINANG;
-      int aw = val - si * NSBINANG;
-      int a = aw / NSBINS, w = aw - a * NSBINS;
-      if (si >= 4084) continue; /* address beyond the span-1 range: skipped */
-      int64_t addr = span == 11 ? (int64_t)sn1_sn11[si] * NSBINANG + aw
-                                : (span == 1 ? (int64_t)val : (int64_t)sn1_ssrb[si] * NSBINANG + aw);
-      int prompt = (word >> 30) & 1;
-      int si_ssrb = sn1_ssrb[si];
+      const int64_t g = base + i;
+      if (btp == 1 && g / LM_PATCH != patch) {
+        patch = g / LM_PATCH;
+        int64_t p0 = patch * LM_PATCH - base, p1 = p0 + LM_PATCH;
+        if (p0 < 0) p0 = 0;
+        if (p1 > n) p1 = n;
+        nev = 0;
+        for (int64_t k = p0; k < p1; k++)
+          if (words[k] > 0) pev[nev++] = words[k];
+      }
       for (int r = 0; r < nrep; r++) {
         uint32_t N = 1;
+        int32_t wr = word;
+        if (btp == 1) {
+          uint32_t c[4] = {(uint32_t)g, (uint32_t)((uint64_t)g >> 32), (uint32_t)r, 1u};
+          philox(c, k0, k1);
+          wr = pev[((uint64_t)c[0] * nev) >> 32];
+        }
+        int32_t val = wr & 0x3fffffff;
+        int si = val / NSBINANG;
+        int aw = val - si * NSBINANG;
+        int a = aw / NSBINS, w = aw - a * NSBINS;
+        if (si >= 4084) continue; /* address beyond the span-1 range: skipped */
+        int64_t addr = span == 11 ? (int64_t)sn1_sn11[si] * NSBINANG + aw
+                                  : (span == 1 ? (int64_t)val : (int64_t)sn1_ssrb[si] * NSBINANG + aw);
+        int prompt = (wr >> 30) & 1;
+        int si_ssrb = sn1_ssrb[si];
         if (btp == 2) {
           uint64_t g = (uint64_t)(base + i);
           uint32_t c[4] = {(uint32_t)g, (uint32_t)(g >> 32), (uint32_t)r, 0u};

@@ -49,6 +49,22 @@ def phantom(seed=1, nsph=20):
     return img
 
 
+def rel_err(got, ref, floor=1e-6):
+    """Largest per-element relative error |got - ref| / max(|ref|, floor *
+    max|ref|) (projection bins or image voxels)."""
+    ref = np.asarray(ref, np.float64)
+    den = np.maximum(np.abs(ref), floor * np.abs(ref).max())
+    return float(np.max(np.abs(np.asarray(got, np.float64) - ref) / den))
+
+
+def margin(name, err, bar):
+    """Print a parity margin (measured error against its bar) and return
+    whether it holds."""
+    print(f'parity {name}: {err:.3g} (bar {bar:g}, margin x{bar / max(err, 1e-300):.3g})',
+          flush=True)
+    return err < bar
+
+
 def gpu_ok():
     try:
         import torch

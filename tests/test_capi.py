@@ -47,6 +47,9 @@ def test_status_codes_without_gpu(L):
     assert L.nipet_osem_subset_sens(None, None, None, None, None, None, None, None, 10, None,
                                     None, None, None) == -1
     assert b'nipet_osem_subset_sens: NULL geometry' in L.nipet_last_error()
+    assert L.nipet_osem_iter(None, None, None, None, None, None, None, None, 1, 2, None, None,
+                             None, None) == -1
+    assert b'nipet_osem_iter: NULL geometry' in L.nipet_last_error()
     assert L.nipet_version().startswith(b'nipet_amd')
     with pytest.raises(_lib.NipetError):
         _lib.check(-1, 'x')
@@ -100,3 +103,26 @@ def test_new_entry_points_reject_bad_arguments(L, params):
     assert L.nipet_varon(None, None, None, 0, 10, None) == -1
     assert L.nipet_rnd_estimate(None, None, 10, None, None, None) == -1
     assert L.nipet_psf_conv(None, None, None, None, 1, None) == -1
+
+
+def test_lm_create_rejects_frame_beyond_acquisition(L, params):
+    """nipet_lm_create refuses tstop > nitag (the head curves and projection
+    views hold nitag seconds) and unknown bootstrap types, before touching the
+    device; LmHistogram checks the frame the same way."""
+    import numpy as np
+
+    from nipet_amd import _lib
+    from nipet_amd.lm.mmrhist import LmHistogram
+    tx, ax = params['txLUT'], params['axLUT']
+    keep = [np.ascontiguousarray(ax['sn1_sn11'], np.int16),
+            np.ascontiguousarray(ax['sn1_ssrb'], np.int16),
+            np.ascontiguousarray(ax['sn1_rno'], np.int16),
+            np.ascontiguousarray(tx['s2cF'], np.int16)]
+    h = ct.c_void_p()
+    for tstart, tstop, btp in ((0, 11, 0), (5, 4, 0), (0, 10, 3)):
+        cfg = _lib.NipetLmConfig(0, 10, tstart, tstop, 11, btp, 1.0, 0, 1, 0,
+                                 *[k.ctypes.data for k in keep])
+        assert L.nipet_lm_create(ct.byref(cfg), ct.byref(h)) == -1
+        assert b'bad configuration' in L.nipet_last_error()
+    with pytest.raises(ValueError, match='outside the acquisition'):
+        LmHistogram(params, 0, 10, 0, 11)

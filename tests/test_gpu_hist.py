@@ -160,3 +160,58 @@ def test_remgaps_u16b(torch, params, stream):
     bad = _lib.lib().nipet_remgaps_u16b(out.data_ptr(), h.out['psino'].data_ptr(),
                                         ali_d.data_ptr(), naw, 837, 86688, 5, 3, 3, None)
     assert bad != 0
+
+
+@pytest.mark.parametrize('nrep', [1, 4])
+def test_hist_nonparametric_bootstrap(torch, params, stream, nrep):
+    """btp = 1 (events redrawn from their 8192-word patch) bit-exact against
+    the oracle, in one chunk and in chunks that start on patch boundaries."""
+    from oracle import oracle as O
+    toff, nitag = _toff_nitag(stream)
+    w = stream[:1_000_000]
+    o = O.lm_hist(w, params, toff, nitag, btp=1, seed=31, nrep=nrep)
+    for cuts in ((len,), (8192 * 3, 8192 * 40, len)):
+        print(f'btp=1 nrep={nrep} chunks={cuts}', flush=True)
+        h, g = _gpu(torch, params, w, toff, nitag, chunks=cuts, btp=1, seed=31, nrep=nrep)
+        _compare(g, o, nrep)
+        assert g['phc'][:, 0].sum() + g['dhc'][:, 0].sum() == o['counted']
+
+
+def test_hist_nonparametric_chunk_alignment(torch, params, stream):
+    """A btp = 1 chunk that does not start on a patch boundary is refused
+    with a status code (the patches are blocks of the session's stream)."""
+    from nipet_amd import _lib
+    from nipet_amd.lm.mmrhist import LmHistogram
+    toff, nitag = _toff_nitag(stream)
+    h = LmHistogram(params, toff, nitag, btp=1, seed=1)
+    wd = torch.from_numpy(stream[:50_000]).cuda()
+    h.add(wd[:8192 * 2])
+    with pytest.raises(_lib.NipetError, match='multiple of 8192'):
+        h.add(wd[:1000])
+        h.add(wd[1000:2000])
+
+
+@pytest.mark.parametrize('btp', [2, 1])
+def test_hist_64_replicates_vs_split_sessions(torch, params, stream, btp):
+    """An nrep = 64 session (the bootstrap batch's histogram) equals 64
+    sessions with nrep = 1 and rep0 = r, replicate by replicate, on every
+    per-replicate output (bit-exact; the single sessions are pinned against
+    the oracle above)."""
+    from nipet_amd.lm.mmrhist import LmHistogram
+    toff, nitag = _toff_nitag(stream)
+    w = torch.from_numpy(stream[:400_000]).cuda()
+    big = LmHistogram(params, toff, nitag, btp=btp, seed=4242, nrep=64)
+    big.add(w)
+    big.finish()
+    for r in range(64):
+        one = LmHistogram(params, toff, nitag, btp=btp, seed=4242, nrep=1, rep0=r)
+        one.add(w)
+        one.finish()
+        for k in ('psino', 'dsino', 'phc', 'dhc'):
+            assert torch.equal(big.out[k][:, r], one.out[k][:, 0]), (r, k)
+        if r == 0:
+            for k in ('ssr', 'zR', 'zM', 'pvs', 'fan', 'bck'):
+                assert torch.equal(big.out[k], one.out[k]), k
+        del one
+    tot = big.out['psino'].sum(dim=0, dtype=torch.int64)
+    assert int((tot != tot[0]).sum()) > 0          # replicates differ

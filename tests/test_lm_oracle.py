@@ -91,3 +91,64 @@ def test_bootstrap_weights_statistics(params):
     assert abs(tot / n - 1.0) < 0.01
     z0 = n - h['counted']                   # N == 0 draws
     assert abs(z0 / n - np.exp(-1)) < 0.005
+
+
+def _patch_stream(words_per_patch, fill, npatch):
+    """Stream of npatch 8192-word patches: time tags at the first word of
+    every patch (1 s apart), then `fill(k)` words (k = patch index)."""
+    out = []
+    for k in range(npatch):
+        p = np.zeros(8192, np.int64)
+        p[0] = TAG | (1000 * k)
+        p[1:1 + words_per_patch] = fill(k)
+        out.append(p)
+    out.append(np.array([TAG | (1000 * npatch)]))
+    return np.concatenate(out).astype(np.int32)
+
+
+def test_nonparametric_bootstrap_stays_in_patch(params):
+    """btp = 1 draws from the 8192-word patch of the event: patches whose
+    events are all one word reproduce the plain histogram exactly (a draw
+    from a neighbouring patch would move counts between bins / prompt and
+    delayed), and each event keeps the time tag of its position."""
+    bins = [(1 << 30) | _addr(5, 3, 7), _addr(900, 100, 200), (1 << 30) | _addr(3000, 250, 1)]
+    w = _patch_stream(5000, lambda k: bins[k % 3], 6)
+    plain = O.lm_hist(w, params, 0, 6)
+    boot = O.lm_hist(w, params, 0, 6, btp=1, seed=3, nrep=2)
+    for r in range(2):
+        for k in ('psino', 'dsino', 'phc', 'dhc'):
+            assert np.array_equal(boot[k][r], plain[k][0]), k
+    for k in ('ssr', 'zR', 'zM', 'pvs', 'fan'):
+        assert np.array_equal(boot[k], plain[k]), k
+
+
+def test_nonparametric_bootstrap_draws(params):
+    """Within a patch the drawn words are uniform over the patch's events
+    (chi-square over 4 distinct words with weights 1:2:3:4), the counts per
+    replicate equal the in-frame event positions, each second keeps its
+    event count (prompts + delayeds), and replicates differ."""
+    words4 = np.array([(1 << 30) | _addr(10, 0, 5), (1 << 30) | _addr(11, 1, 6),
+                       _addr(12, 2, 7), (1 << 30) | _addr(13, 3, 8)], np.int64)
+    rng = np.random.default_rng(0)
+
+    def fill(k):
+        return rng.permutation(np.repeat(words4, [800, 1600, 2400, 3200]))
+
+    w = _patch_stream(8000, fill, 8)
+    nev = 8 * 8000
+    h = O.lm_hist(w, params, 0, 8, btp=1, seed=77, nrep=3)
+    plain = O.lm_hist(w, params, 0, 8)
+    s11 = params['axLUT']['sn1_sn11'].astype(np.int64)
+    for r in range(3):
+        got = []
+        for x in words4:
+            v = int(x) & 0x3fffffff
+            si, aw = v // 86688, v % 86688
+            arr = h['psino'] if (int(x) >> 30) & 1 else h['dsino']
+            got.append(int(arr[r][s11[si] * 86688 + aw]))
+        assert sum(got) == nev
+        exp = nev * np.array([1, 2, 3, 4]) / 10
+        chi2 = float(((np.array(got) - exp) ** 2 / exp).sum())
+        assert chi2 < 16.3, (got, chi2)          # 3 dof, p = 0.001
+        assert np.array_equal(h['phc'][r] + h['dhc'][r], plain['phc'][0] + plain['dhc'][0])
+    assert not np.array_equal(h['psino'][0], h['psino'][1])

@@ -50,3 +50,50 @@ def test_half_lives_in_seconds():
     assert 6500 < hl['F18'] < 6700 and 1200 < hl['C11'] < 1250 and 120 < hl['O15'] < 125
     assert 4000 < hl['Ga68'] < 4100 and 2.3e7 < hl['Ge68'] < 2.4e7
     assert all(0.85 < v['BF'] <= 1.0 for v in RI_LUT.values())
+
+
+def test_prompts_u16_saturates_for_every_input_type():
+    """osemone's prompts reach the device as uint16 by one rule whatever their
+    container: counts saturate at 0 and 65535, fractions truncate (NumPy
+    arrays and torch tensors, integer and float)."""
+    import torch
+
+    from nipet_amd.prj.osem import prompts_u16
+    vals = [0, 1, 65535, 65536, 70000, 2**31 - 1, -3]
+    exp = [0, 1, 65535, 65535, 65535, 65535, 0]
+    for dt in (np.int64, np.int32, np.uint32):
+        v = np.array([x for x in vals if dt != np.uint32 or x >= 0], dt)
+        e = [y for x, y in zip(vals, exp) if dt != np.uint32 or x >= 0]
+        assert list(prompts_u16(v)) == e
+        assert prompts_u16(v).dtype == np.uint16
+        t = prompts_u16(torch.from_numpy(v.astype(np.int64)))
+        assert t.dtype == torch.uint16 and t.to(torch.int64).tolist() == e
+    f = np.array([12.7, 65535.9, 1e9, -0.5], np.float32)
+    assert list(prompts_u16(f)) == [12, 65535, 65535, 0]
+    assert prompts_u16(torch.from_numpy(f)).to(torch.int64).tolist() == [12, 65535, 65535, 0]
+    assert list(prompts_u16(np.array([5, 6], np.int16))) == [5, 6]
+    assert prompts_u16(torch.tensor([5, -6], dtype=torch.int16)).to(torch.int64).tolist() == [5, 0]
+
+
+def test_frame_definitions_and_skip_rule(params):
+    """mmrchain's frame forms (img/pipe.py:95-125, dynamic_timings
+    lm/mmrhist.py:492-556) and the skip of frames whose delayeds exceed 0.99
+    of the prompts (img/pipe.py:317-324), which needs no reconstruction."""
+    import nipet_amd as nip
+    from nipet_amd.img.pipe import frame_timings
+    assert frame_timings(['fluid', [0, 10], [10, 40]]) == [[0, 10], [10, 40]]
+    assert frame_timings(['timings', [5, 6]]) == [[5, 6]]
+    assert frame_timings(['def', [2, 15], [1, 30]]) == [[0, 15], [15, 30], [30, 60]]
+    assert frame_timings([10, 20, 30]) == [[0, 10], [10, 30], [30, 60]]
+    dt = nip.dynamic_timings([15, 30], offset=10)
+    assert dt['timings'] == ['timings', [0, 10], [10, 25], [25, 55]] and dt['total'] == 55
+    assert list(dt['frames']) == [15, 30]
+    assert nip.dynamic_timings(['def', [2, 15]], offset=10)['timings'][1:] == [[0, 15], [15, 30]]
+    with pytest.raises(TypeError):
+        frame_timings(['def', [1, 2, 3]])
+    hst = {'psino': np.zeros((837, 252, 344), np.uint16), 't0': 0, 't1': 10,
+           'phc': np.full(10, 100), 'dhc': np.full(10, 100)}
+    out = nip.mmrchain({}, params, histo=hst, itr=1)
+    assert out['frames'] == [[0, 10]] and out['#frames'] == 1 and out['recmod'] == 0
+    assert out['im'].shape == (127, 344, 344) and not out['im'].any()
+    assert 'tuple' not in out
