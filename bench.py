@@ -15,17 +15,28 @@ with a bone shell 0.15/cm; prompts = Poisson(scale (a n FP(phantom)) + randoms
 + scatter) at 3e8 expected counts; norm U(0.8, 1.2); randoms flat 12 % and
 scatter smooth 25 % of the trues.
 
-Batched bootstrap (extra field "batched", skipped by --no-batch): a synthetic
-600 s list-mode stream of 6e8 words (1 ms tags, 85 % prompts drawn from
-a n FP(phantom), 15 % delayeds) is histogrammed once into 64 Poisson(1)
-bootstrap replicates and reconstructed 4x14 as one batch (one HIP graph
-launch, autotuned sub-batch widths); it is compared with plain
-reconstructions of single replicates (one histogram session with that
-replicate, osem_prepare, osem_run), whose images it must equal. The same
-stream is cut into 24 dynamic frames of 10-300 s (extra field "frames"),
-reconstructed as one batch against plain per-frame reconstructions; the
+Roofline (field "roofline"): the dominant kernel's HBM throughput from its
+rocprofv3-measured DRAM bytes per launch (profiles/pmc_<kernel>_osem4x14.json)
+over its live mean launch time (HIP events on the library's stream), against
+8 TB/s; the SQ instruction counters of the same kernel
+(profiles/sq_<kernel>_osem4x14.json) over the same time give the issue rates
+that actually bind it; the per-term byte model is reported as model_GBs.
+
+Batched bootstrap + dynamic frames (extra field "batched", skipped by
+--no-batch): a synthetic 600 s list-mode stream of 6e8 words (1 ms tags, 85 %
+prompts drawn from the expected prompts a n FP(phantom) + randoms + scatter,
+15 % delayeds) is histogrammed once into 64 Poisson(1) bootstrap replicates
+and cut into 24 dynamic frames of 10-300 s; all 88 are reconstructed 4x14 as
+one batch (one HIP graph launch, autotuned sub-batch widths) and compared
+with plain reconstructions of single replicates and frames (one histogram
+session, osem_prepare, osem_run each), whose images they must equal. The
 "histogram" field times the histogramming of the stream alone (plain and with
 64 bootstrap replicates).
+
+CPU baseline (field "cpu_baseline", skipped by --no-cpu): the float32 OpenMP
+transcription of the reference's tracer on the host cores, one full subset
+FP + BP pair (median of 3 after a warm-up), plus the reference's histogram
+loop over the first 3e7 words of the stream.
 
 Autograd (extra field "autograd", --map-steps 0 skips): 200 gradient MAP steps
 of the Poisson likelihood + quadratic prior through the differentiable
@@ -36,6 +47,7 @@ fan-sums, the normalisation sinogram from components and one PSF convolution
 of the phantom, each timed with HIP events.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch]
+                       [--no-corr] [--map-steps N]
 """
 import argparse
 import json
@@ -48,6 +60,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "FP+BP pairs/s & % HBM peak (mMR span-11); batched bootstrap recons/s vs plain"
 HBM_PEAK_GBS = 8000.0
+CLOCK_HZ, N_CU = 2.4e9, 256   # MI355X peak engine clock, compute units
 NITER, NSUB = 4, 14
 
 
@@ -64,15 +77,16 @@ def parse():
     ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
-    ap.add_argument('--no-batch', action='store_true', help='skip the batched bootstrap leg')
+    ap.add_argument('--no-batch', action='store_true',
+                    help='skip the list-mode legs (histogram and batched reconstruction)')
+    ap.add_argument('--no-recon', action='store_true',
+                    help='list-mode legs: histogram only, no batched reconstruction')
     ap.add_argument('--map-steps', type=int, default=200,
                     help='gradient MAP steps of the autograd leg (0: skip)')
     ap.add_argument('--no-corr', action='store_true',
                     help='skip the randoms / normalisation / PSF timing leg')
     ap.add_argument('--nrep', type=int, default=64, help='bootstrap replicates of the batch')
     ap.add_argument('--lm-words', type=float, default=6e8, help='list-mode words (600 s)')
-    ap.add_argument('--cpu-seconds', type=float, default=12.0,
-                    help='approximate CPU work of the baseline sample')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='BYTES',
                     help='only run a plain device copy of BYTES (rocprof counter scaling)')
     return ap.parse_args()
@@ -89,32 +103,60 @@ def traffic_per_launch(kernel):
         return json.load(f).get('hbm_bytes_per_launch')
 
 
-def cpu_baseline(params, budget_s):
-    """FP64 oracle FP+BP over subset 0 on the host cores, repeated on growing
-    ray samples until about budget_s seconds of CPU work; pairs/s of a full
-    subset."""
+def cpu_baseline(params, words=None, runs=3):
+    """The float32 OpenMP transcription of the reference's tracer
+    (oracle/cpu_tracer.cpp: sddn_tx tables, fprj/bprj_drct+oblq with
+    per-thread BP images) timed on the host cores: one FP + BP pair over all
+    4,894 bins of subset 0 of a random image, one warm-up pair then the median
+    of `runs`, as pairs/s. Beside it, the reference's histogram loop (hst)
+    over a bounded sample of the list-mode stream (words/s), and the FP64
+    oracle over a few subset-0 rays (information only)."""
     import numpy as np
 
     import nipet_amd as nip
+    from oracle import cpu_tracer as C
     from oracle import oracle as O
     sub = nip.get_subsets14(0, params)[0]
     rng = np.random.default_rng(0)
     img = rng.random((320, 320, 127), dtype=np.float32)
-    rays_done, dt = 0, 0.0
-    n = 512
-    while dt < budget_s:
-        rows = np.sort(rng.choice(sub, min(n, len(sub)), replace=False)).astype(np.int32)
-        t0 = time.perf_counter()
-        y = O.fprj(params, img, rows)
-        O.bprj(params, y, rows)
-        dt += time.perf_counter() - t0
-        rays_done += len(rows)
-        n *= 2
-    pairs_s = rays_done / (dt * len(sub))
-    return {'value': pairs_s, 'unit': 'pairs/s', 'cores': int(O.lib().orc_max_threads()),
-            'kind': 'port',
-            'sample': f'FP64 oracle FP+BP (emission) over {rays_done} subset-0 rays in '
-                      f'{dt:.1f} s, scaled to one full-subset pair ({len(sub)} rays)'}
+    T = C.Tracer(params)
+    fp, bp = T.time_pair(img, sub, runs=runs)
+    t_pair = float(np.median(fp + bp))
+    info = C.host_info()
+    out = {'value': round(1.0 / t_pair, 4), 'unit': 'pairs/s', 'cores': info['omp_threads'],
+           'kind': 'port',
+           'sample': f'float32 OpenMP transcription of the reference tracer: FP + BP over all '
+                     f'{len(sub)} bins x 837 sinograms of subset 0 (one pair), warm-up + median '
+                     f'of {runs}; FP {np.median(fp):.3f} s, BP {np.median(bp):.3f} s',
+           'nproc': info['nproc'], 'cpu_model': info['cpu_model']}
+    if words is not None:
+        from nipet_amd.lm.mmrhist import tag_range
+        nw = min(int(words.numel()), 30_000_000)
+        w = words[:nw].cpu().numpy()
+        tf, tl = tag_range(w)
+        nitag = ((tl - tf) + 999) // 1000 + 1
+        secs, counted = C.time_hst(w, params, tf, nitag, runs=runs)
+        out['histogram'] = {'words_per_s': round(nw / float(np.median(secs)), 1),
+                            'sample': f'hst event loop over the first {nw} words '
+                                      f'({counted} events counted), warm-up + median of {runs}'}
+    rows = np.sort(rng.choice(sub, 256, replace=False)).astype(np.int32)
+    t0 = time.perf_counter()
+    O.bprj(params, O.fprj(params, img, rows), rows)
+    dt = time.perf_counter() - t0
+    out['oracle_fp64_info'] = {'pairs_per_s': round(len(rows) / (dt * len(sub)), 4),
+                               'sample': f'FP64 oracle over {len(rows)} subset-0 rays (not the '
+                                         'baseline)'}
+    return out
+
+
+def sq_counters(kernel):
+    """Per-launch SQ counters of `kernel` over the full OSEM (rocprofv3 --pmc),
+    read from profiles/sq_<kernel>_osem4x14.json when that summary exists."""
+    path = os.path.join(ROOT, 'profiles', f'sq_{kernel}_osem4x14.json')
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
 
 
 def copy_calibration(nbytes):
@@ -135,19 +177,17 @@ LM_SECONDS = 600
 
 
 def lm_stream(params, d, nwords):
-    """Synthetic 600 s list-mode stream on the device whose prompts follow
-    a n FP(phantom)."""
+    """Synthetic 600 s list-mode stream on the device whose prompt bins follow
+    the expected prompts of the OSEM data set (a n FP(phantom) + randoms +
+    scatter); returns (words, randoms + scatter at the stream's count level)."""
     import torch
 
-    from nipet_amd.lm.lmsim import synth_lm
-    from nipet_amd.prj.mmrprj import frwd_prj
+    from nipet_amd.prj.mmrsim import lm_workload
     per_ms = max(1, int(round(nwords / (LM_SECONDS * 1000))))
-    emis = frwd_prj(torch.from_numpy(d['img']).cuda(), params, dev_out=True)
     log(f'synthesising {LM_SECONDS} s list-mode stream ({per_ms} events/ms)')
-    words = synth_lm(params, LM_SECONDS, per_ms, weights=d['attn'] * d['norm'] * emis, seed=5)
-    del emis
+    words, rs = lm_workload(params, d, seconds=LM_SECONDS, per_ms=per_ms, seed=5)
     torch.cuda.synchronize()
-    return words
+    return words, rs
 
 
 def histogram(params, words, nrep=64):
@@ -200,78 +240,36 @@ def map_leg(params, d, steps):
             'timing': 'wall clock incl. attenuation FP and sensitivity BP of the set-up'}
 
 
-# 24 dynamic frames of uneven length (10-300 s) covering the 600 s stream
-FRAMES = ([(10 * i, 10 * i + 10) for i in range(14)] +
-          [(140 + 15 * i, 155 + 15 * i) for i in range(4)] +
-          [(200 + 20 * i, 220 + 20 * i) for i in range(5)] +
-          [(300, 600)])
+def _rel_l2(torch, a, b):
+    return float(torch.linalg.vector_norm((a - b).double()) / torch.linalg.vector_norm(b.double()))
 
 
-def batched_frames(params, d, words, nplain=2):
-    """24 dynamic frames (10-300 s) reconstructed 4x14 in one batch (one graph
-    launch, autotuned widths) vs plain per-frame reconstructions of the
-    shortest and the longest frame."""
-    import torch
-
-    from nipet_amd.prj.batch import frame_prompts, frames_osem
-    from nipet_amd.prj.osem import osem_prepare, osem_run
-    rs = (d['randoms'] + d['scatter']) * 1e-3
-    rho = torch.stack([rs * ((t1 - t0) / LM_SECONDS) for t0, t1 in FRAMES], dim=2).contiguous()
-    torch.cuda.synchronize()
-    log(f'batched frames: {len(FRAMES)}')
-    t0 = time.perf_counter()
-    imgs, info = frames_osem(words, params, FRAMES, d['attn'], d['norm'], rho, itr=NITER)
-    torch.cuda.synchronize()
-    t_batch = time.perf_counter() - t0
-    t_plain, rels = [], []
-    for k in range(nplain):
-        f = 0 if k == 0 else (len(FRAMES) - 1 if k == 1 else (k * 13) % len(FRAMES))
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        p, _ = frame_prompts(words, params, [FRAMES[f]])
-        st = osem_prepare(params, p[..., 0].contiguous(), d['attn'], d['norm'],
-                          rho[..., f].contiguous())
-        img = osem_run(st, NITER, check_counts=False)
-        torch.cuda.synchronize()
-        t_plain.append(time.perf_counter() - t0)
-        rels.append(float(torch.linalg.vector_norm((imgs[..., f] - img).double()) /
-                          torch.linalg.vector_norm(img.double())))
-        del st, img, p
-    tp = sum(t_plain) / len(t_plain)
-    nf = len(FRAMES)
-    log(f'frames: batched {t_batch:.2f} s for {nf}, plain {tp:.3f} s per frame')
-    return {'workload': f'{nf} dynamic frames of 10-300 s of the {LM_SECONDS} s stream, OSEM '
-                        f'{NITER}x{NSUB} each, span-11, per-frame randoms+scatter',
-            'recons_per_s': round(nf / t_batch, 4), 'plain_recons_per_s': round(1.0 / tp, 4),
-            'speedup_vs_plain': round((nf / t_batch) * tp, 3), 'batch_s': round(t_batch, 3),
-            'plain_checked': nplain, 'max_rel_l2_vs_plain': max(rels),
-            'widths': {'fprj': info['tune']['fprj'], 'bprj': info['tune']['bprj']},
-            'timing': 'wall clock incl. per-frame histogramming, corrections, sensitivity, '
-                      'autotune, graph capture'}
-
-
-def batched_bootstrap(params, d, words, nrep, nplain=2):
-    """Batched vs plain bootstrap reconstructions (see the module docstring)."""
+def batched(params, d, words, rs, nrep):
+    """64 Poisson(1) bootstrap replicates of the 600 s stream and its 24
+    dynamic frames (10-300 s) reconstructed 4x14 as ONE batch (one list-mode
+    pass for the replicates, one session per frame, one HIP graph launch for
+    every subset update), against plain reconstructions of single replicates
+    (one histogram session with that replicate, osem_prepare, osem_run) and
+    of single frames (one frame session, osem_prepare, osem_run), whose images
+    the batch must equal. Randoms + scatter: the stream's own (12 % + 25 % of
+    the trues), scaled by duration per frame."""
+    import numpy as np
     import torch
 
     from nipet_amd.lm.mmrhist import hist_words
-    from nipet_amd.prj.batch import _remgaps_b, bootstrap_osem
+    from nipet_amd.prj.batch import FRAMES24, _remgaps_b, frame_prompts, replicates_and_frames
     from nipet_amd.prj.osem import osem_prepare, osem_run
-    dur = LM_SECONDS
-    rs = (d['randoms'] + d['scatter']) * 1e-3
     seed = 2024
     torch.cuda.synchronize()
-    log(f'{words.numel()} words; batched bootstrap of {nrep} replicates')
+    log(f'batch: {nrep} replicates + {len(FRAMES24)} frames, one graph')
     t0 = time.perf_counter()
-    imgs, info = bootstrap_osem(words, params, d['attn'], d['norm'], rs, nrep=nrep, seed=seed,
-                                itr=NITER)
+    reps, frms, info = replicates_and_frames(words, params, d['attn'], d['norm'], rs, nrep=nrep,
+                                             seed=seed, itr=NITER)
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
-    log(f'batched: {t_batch:.2f} s for {nrep}; widths {info["tune"]["fprj"]}/'
-        f'{info["tune"]["bprj"]}')
-    t_plain, rels = [], []
-    for r in range(nplain):
-        rep = (r * 37) % nrep
+    log(f'batch: {t_batch:.2f} s; {info["timing"]}')
+    t_rep, t_frm, rels = [], [], {}
+    for rep in (0, nrep - 1):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         h = hist_words(words, params, btp=2, seed=seed, nrep=1, rep0=rep)
@@ -280,23 +278,41 @@ def batched_bootstrap(params, d, words, nrep, nplain=2):
         st = osem_prepare(params, p, d['attn'], d['norm'], rs)
         img = osem_run(st, NITER, check_counts=False)
         torch.cuda.synchronize()
-        t_plain.append(time.perf_counter() - t0)
-        rels.append(float(torch.linalg.vector_norm((imgs[..., rep] - img).double()) /
-                          torch.linalg.vector_norm(img.double())))
+        t_rep.append(time.perf_counter() - t0)
+        rels[f'replicate_{rep}'] = _rel_l2(torch, reps[rep], img)
         del st, img, p
-    tp = sum(t_plain) / len(t_plain)
-    log(f'plain: {tp:.3f} s per replicate; max rel L2 {max(rels):.3g}')
+    lengths = [t1 - t0 for t0, t1 in FRAMES24]
+    dur = LM_SECONDS
+    for f in (int(np.argmin(lengths)), int(np.argmax(lengths))):
+        fa, fb = FRAMES24[f]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        p, _ = frame_prompts(words, params, [FRAMES24[f]])
+        st = osem_prepare(params, p[..., 0].contiguous(), d['attn'], d['norm'],
+                          rs * ((fb - fa) / dur))
+        img = osem_run(st, NITER, check_counts=False)
+        torch.cuda.synchronize()
+        t_frm.append(time.perf_counter() - t0)
+        rels[f'frame_{f}'] = _rel_l2(torch, frms[f], img)
+        del st, img, p
+    tr, tf = sum(t_rep) / len(t_rep), sum(t_frm) / len(t_frm)
+    n = nrep + len(FRAMES24)
+    t_seq = nrep * tr + len(FRAMES24) * tf
+    log(f'plain: {tr:.3f} s per replicate, {tf:.3f} s per frame; rel L2 {rels}')
+    del info['graph']
     return {'workload': f'{nrep} Poisson(1) bootstrap replicates of a {dur} s list-mode stream '
-                        f'of {words.numel()} words, OSEM {NITER}x{NSUB} each, span-11',
-            'recons_per_s': round(nrep / t_batch, 4),
-            'plain_recons_per_s': round(1.0 / tp, 4),
-            'speedup_vs_plain': round((nrep / t_batch) * tp, 3),
-            'batch_s': round(t_batch, 3), 'plain_s_per_recon': round(tp, 4),
-            'plain_checked': nplain, 'max_rel_l2_vs_plain': max(rels),
+                        f'of {words.numel()} words + its {len(FRAMES24)} dynamic frames of '
+                        f'10-300 s, OSEM {NITER}x{NSUB} each, span-11, one graph launch',
+            'recons_per_s': round(n / t_batch, 4), 'plain_recons_per_s': round(n / t_seq, 4),
+            'speedup_vs_plain': round(t_seq / t_batch, 3), 'batch_s': round(t_batch, 3),
+            'plain_s_per_replicate': round(tr, 4), 'plain_s_per_frame': round(tf, 4),
+            'plain_checked': len(rels), 'max_rel_l2_vs_plain': max(rels.values()),
+            'rel_l2_vs_plain': rels,
             'widths': {'fprj': info['tune']['fprj'], 'bprj': info['tune']['bprj']},
             'batch_breakdown_s': info['timing'],
-            'timing': 'wall clock incl. histogramming, gap removal, corrections, '
-                      'sensitivity, autotune, graph capture'}
+            'timing': 'wall clock incl. histogramming, gap removal, corrections, sensitivity, '
+                      'autotune, graph capture; plain = sequential estimate from the timed '
+                      'plain runs (replicates and frames separately)'}
 
 
 def _event_ms(fn, reps):
@@ -416,14 +432,37 @@ def main():
     t_bp = kt.mean_ms(_lib.PROF_BPRJ) * 1e-3
     t_fp = kt.mean_ms(_lib.PROF_FPRJ) * 1e-3
     dominant, b_dom, t_dom = ('k_bprj', b_bp, t_bp) if t_bp >= t_fp else ('k_fprj', b_fp, t_fp)
-    achieved = b_dom / t_dom / 1e9
-    roofline = {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS,
-                'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
-                'traffic': traffic_per_launch(dominant), 'kernel': dominant,
-                'bytes_model': 'BP 8I+4M, FP 4I+4M per subset launch',
+    # HBM roofline of the dominant kernel from its measured DRAM bytes per
+    # launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated); the
+    # per-term byte model (every (LOR, voxel) term an HBM access) is reported
+    # beside it: it counts LDS and L2 hits and exceeds the peak
+    traffic = traffic_per_launch(dominant)
+    achieved = traffic / t_dom / 1e9 if traffic else None
+    sq = sq_counters(dominant)
+    issue = None
+    if sq:
+        cyc = t_dom * CLOCK_HZ * N_CU   # CU-cycles of one launch
+        issue = {'clock_GHz': CLOCK_HZ / 1e9, 'cus': N_CU,
+                 'valu_per_cu_cycle': round(sq['SQ_INSTS_VALU'] / cyc, 3),
+                 'salu_per_cu_cycle': round(sq['SQ_INSTS_SALU'] / cyc, 3),
+                 'lds_idx_active_per_cu_cycle': round(sq['SQ_LDS_IDX_ACTIVE'] / cyc, 3),
+                 'lds_bank_conflict_frac': round(sq['SQ_LDS_BANK_CONFLICT'] /
+                                                 sq['SQ_LDS_IDX_ACTIVE'], 3),
+                 'note': 'SQ counters per launch from the committed rocprofv3 --pmc summary '
+                         'over this live launch time; one VALU (wave64) and one SALU '
+                         'instruction per CU-cycle is the issue limit'}
+    roofline = {'bound': 'hbm', 'achieved': round(achieved, 1) if achieved else None,
+                'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                'frac': round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                'traffic': traffic, 'kernel': dominant, 'binding': 'instruction issue',
+                'issue': issue,
+                'model_GBs': round(b_dom / t_dom / 1e9, 1),
+                'bytes_model': 'BP 8I+4M, FP 4I+4M per subset launch (I = (LOR, voxel) '
+                               'terms, M = bins x sinograms): every term counted as HBM',
                 'I_per_subset': float(terms.mean()),
                 'fp_ms': round(t_fp * 1e3, 4), 'bp_ms': round(t_bp * 1e3, 4),
-                'fp_GBs': round(b_fp / t_fp / 1e9, 1), 'bp_GBs': round(b_bp / t_bp / 1e9, 1),
+                'fp_model_GBs': round(b_fp / t_fp / 1e9, 1),
+                'bp_model_GBs': round(b_bp / t_bp / 1e9, 1),
                 'update_ms': round(kt.mean_ms(_lib.PROF_OSEM_UPDATE), 4),
                 # first iteration: sensitivity and ratio back-projected together
                 'bp_pair_ms': round(kt.mean_ms(_lib.PROF_BPRJ_PAIR), 4)}
@@ -443,17 +482,19 @@ def main():
     if not args.no_corr:
         log('corrections')
         out['corrections'] = corrections(params, d)
+    words = None
     if not args.no_batch:
-        words = lm_stream(params, d, args.lm_words)
+        words, rs = lm_stream(params, d, args.lm_words)
         out['histogram'] = histogram(params, words, args.nrep)
-        out['batched'] = batched_bootstrap(params, d, words, args.nrep)
-        out['frames'] = batched_frames(params, d, words)
-        del words
+        if not args.no_recon:
+            out['batched'] = batched(params, d, words, rs, args.nrep)
+        del rs
     if args.map_steps > 0:
         out['autograd'] = map_leg(params, d, args.map_steps)
     if not args.no_cpu:
         log('CPU baseline')
-        out['cpu_baseline'] = cpu_baseline(params, args.cpu_seconds)
+        out['cpu_baseline'] = cpu_baseline(params, words)
+    del words
     print(json.dumps(out), flush=True)
 
 

@@ -127,6 +127,32 @@ def test_mmrhist_file(torch, params, stream, tmp_path):
     assert hst['fansums'].shape == (64, 504) and hst['pssr'].shape == (127, 252, 344)
 
 
+def test_mmrhist_sinogram_cache(torch, params, stream, tmp_path):
+    """store=True writes the raw histogram of the frame as
+    sino/sinos_s11_frm-t0-t1.npz under outpath; use_stored=True with
+    datain['sinos'] naming that file returns the same outputs without the
+    list-mode file, and a cache file of another frame is not used
+    (lm/mmrhist.py:65-69, 116-125)."""
+    import nipet_amd as nip
+    f = os.path.join(tmp_path, 'lm.bf')
+    stream.tofile(f)
+    out = os.path.join(tmp_path, 'out')
+    hst = nip.mmrhist({'lm_bf': f}, params, t0=1, t1=5, store=True, outpath=out)
+    cache = os.path.join(out, 'sino', 'sinos_s11_frm-1-5.npz')
+    assert os.path.isfile(cache)
+    os.remove(f)
+    again = nip.mmrhist({'sinos': cache, 'lm_bf': f}, params, t0=1, t1=5, use_stored=True)
+    assert again.keys() == hst.keys()
+    for k, v in hst.items():
+        if isinstance(v, np.ndarray):
+            assert v.dtype == again[k].dtype and np.array_equal(v, again[k]), k
+        else:
+            assert v == again[k], k
+    # the cached frame is 1-5 s: another frame falls back to the (now missing) file
+    with pytest.raises(ValueError):
+        nip.mmrhist({'sinos': cache, 'lm_bf': f}, params, t0=0, t1=5, use_stored=True)
+
+
 def test_rep0_selects_replicate(torch, params, stream):
     """A session with nrep = 1, rep0 = 2 histograms exactly replicate 2 of an
     nrep = 3 session (oracle), so replicate sets can be split across sessions."""
