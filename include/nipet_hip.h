@@ -106,6 +106,19 @@ int nipet_remgaps(float *out, const void *in, int in_dtype, const int32_t *aw2al
 int nipet_remgaps_u16b(uint16_t *out, const uint32_t *in, const int32_t *aw2ali, int naw,
                        int nsn, int nbinang, int nrep, int r0, int nb, void *stream);
 
+/* nipet_remgaps without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight per-frame loops, harnesses
+ * that forbid hipMalloc). No counterpart in the reference. The inverse bin
+ * map lives in a static device array of NIPET_NOALLOC_MAXBINANG int32
+ * (347 KB, held per device until the process ends); nbinang beyond that is
+ * rejected with NIPET_EINVAL. Every call on one device shares that array and
+ * rewrites it whole: calls that overlap on one device (different streams or
+ * threads) race. Same arguments and result as nipet_remgaps. */
+#define NIPET_NOALLOC_MAXBINANG (252 * 344)
+#define NIPET_MAXDEVICES 64
+int nipet_remgaps_noalloc(float *out, const void *in, int in_dtype, const int32_t *aw2ali,
+                          int naw, int nsn, int nbinang, void *stream);
+
 /* Gap insertion: device sinogram in[naw][nsn] -> Siemens out[nsn][nbinang]
  * (gaps zero), out[s][aw2ali[aw]] = in[aw][s]. Replaces mmr_auxe.pgaps /
  * put_gaps (src/scanner_0.cu:158-241) and the transpose of mmraux.py:1141. */

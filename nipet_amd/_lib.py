@@ -78,6 +78,7 @@ def lib():
             'nipet_bprj': ([P, P, P, P, I, P], I),
             'nipet_osem_corr': ([P, P, P, P, P, P, I64, P], I),
             'nipet_remgaps': ([P, P, I, P, I, I, I, P], I),
+            'nipet_remgaps_noalloc': ([P, P, I, P, I, I, I, P], I),
             'nipet_putgaps': ([P, P, P, I, I, I, P], I),
             'nipet_remgaps_u16b': ([P, P, P, I, I, I, I, I, I, P], I),
             'nipet_tune_set': ([I, I], I),

@@ -17,6 +17,7 @@
 //    (segment, plane) instead of one per (LOR, piece) (prj/src/prjb.cu:70-82,
 //    153-185).
 #include <atomic>
+#include <mutex>
 #include <utility>
 
 #include "nipet_common.h"
@@ -1205,6 +1206,70 @@ int nipet_remgaps(float *out, const void *in, int in_dtype, const int32_t *aw2al
     k_remgaps_t<uint16_t><<<grid, 256, 0, st>>>(out, (const uint16_t *)in, inv, nsn, nbinang);
   NIPET_HIP(hipGetLastError());
   NIPET_HIP(hipFreeAsync(inv, st));
+  return NIPET_OK;
+}
+
+// ---- allocation-free gap removal ----
+// The inverse bin map of nipet_remgaps lives in a file-scope device array
+// instead of stream-ordered scratch. Host code never passes `scratch` itself
+// to a kernel (in host code that name is a host-side stand-in, not a device
+// address): the address is looked up once per device with
+// hipGetSymbolAddress, with that device current, and kept in g_scratch_dev.
+static __device__ __align__(256) int32_t scratch[NIPET_NOALLOC_MAXBINANG];
+
+static std::mutex g_scratch_mu;
+static int32_t *g_scratch_dev[NIPET_MAXDEVICES] = {};
+
+static int scratch_ptr(int32_t **out, const char *fn) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, fn);
+  if (dev < 0 || dev >= NIPET_MAXDEVICES) {
+    set_error("%s: device %d beyond the %d devices this library supports", fn, dev,
+              NIPET_MAXDEVICES);
+    return NIPET_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("%s: hipGetSymbolAddress(scratch) failed on device %d: %s", fn, dev,
+                hipGetErrorString(e));
+      return NIPET_EHIP;
+    }
+    g_scratch_dev[dev] = (int32_t *)p;
+  }
+  *out = g_scratch_dev[dev];
+  return NIPET_OK;
+}
+
+int nipet_remgaps_noalloc(float *out, const void *in, int in_dtype, const int32_t *aw2ali,
+                          int naw, int nsn, int nbinang, void *stream) {
+  if (!out || !in || !aw2ali || naw <= 0 || nsn <= 0 || nbinang < naw ||
+      (in_dtype != 0 && in_dtype != 1)) {
+    set_error("nipet_remgaps_noalloc: bad argument");
+    return NIPET_EINVAL;
+  }
+  if (nbinang > NIPET_NOALLOC_MAXBINANG) {
+    set_error("nipet_remgaps_noalloc: nbinang = %d exceeds the static scratch of %d bins",
+              nbinang, NIPET_NOALLOC_MAXBINANG);
+    return NIPET_EINVAL;
+  }
+  int32_t *inv = nullptr;
+  int rc = scratch_ptr(&inv, "nipet_remgaps_noalloc");
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  // the whole map is rewritten by a kernel in every call (no memset node, so
+  // the call may be captured into a graph)
+  k_gap_inv<<<(nbinang + 255) / 256, 256, 0, st>>>(inv, nbinang);
+  k_gap_inv_set<<<(naw + 255) / 256, 256, 0, st>>>(inv, aw2ali, naw, nbinang);
+  const dim3 grid((nbinang + GT - 1) / GT, (nsn + GT - 1) / GT);
+  if (in_dtype == 0)
+    k_remgaps_t<float><<<grid, 256, 0, st>>>(out, (const float *)in, inv, nsn, nbinang);
+  else
+    k_remgaps_t<uint16_t><<<grid, 256, 0, st>>>(out, (const uint16_t *)in, inv, nsn, nbinang);
+  NIPET_HIP(hipGetLastError());
   return NIPET_OK;
 }
 

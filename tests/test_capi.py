@@ -50,6 +50,12 @@ def test_status_codes_without_gpu(L):
     assert L.nipet_osem_iter(None, None, None, None, None, None, None, None, 1, 2, None, None,
                              None, None) == -1
     assert b'nipet_osem_iter: NULL geometry' in L.nipet_last_error()
+    # the allocation-free gap removal refuses layouts beyond its static array
+    # before touching the device
+    buf = ct.c_float()
+    assert L.nipet_remgaps_noalloc(ct.byref(buf), ct.byref(buf), 0, ct.byref(buf), 4, 4,
+                                   252 * 344 + 1, None) == -1
+    assert b'exceeds the static scratch' in L.nipet_last_error()
     assert L.nipet_version().startswith(b'nipet_amd')
     with pytest.raises(_lib.NipetError):
         _lib.check(-1, 'x')
