@@ -145,7 +145,10 @@ def test_mmrhist_sinogram_cache(torch, params, stream, tmp_path):
     assert again.keys() == hst.keys()
     for k, v in hst.items():
         if isinstance(v, np.ndarray):
-            assert v.dtype == again[k].dtype and np.array_equal(v, again[k]), k
+            # cmass is NaN wherever a second holds no prompt (zR / zM = 0 / 0, as
+            # the reference computes it)
+            assert v.dtype == again[k].dtype, k
+            assert np.array_equal(v, again[k], equal_nan=v.dtype.kind == 'f'), k
         else:
             assert v == again[k], k
     # the cached frame is 1-5 s: another frame falls back to the (now missing) file
