@@ -99,12 +99,14 @@ int nipet_remgaps(float *out, const void *in, int in_dtype, const int32_t *aw2al
 
 /* Gap removal of a replicate-innermost histogram: in[(s * nbinang + ali) * nrep
  * + r] (uint32 counts, the layout nipet_lm_chunk writes for nrep replicates)
- * -> out[(aw * nsn + s) * nb + b] = min(in[.. r0 + b], 65535) as uint16, the
- * batched prompts of nipet_osem_subset_b. The reference removes gaps per
- * frame (mmraux.py:1146-1160, src/scanner_0.cu:94-155) and hands petprj.osem
- * uint16 prompts (prj/src/prj_module.cu:649-651). */
+ * -> out[(aw * nsn + s) * out_nb + out_b0 + b] = min(in[.. r0 + b], 65535) as
+ * uint16, b < nb (out_b0 + nb <= out_nb): columns out_b0 .. out_b0+nb-1 of the
+ * batched prompts of nipet_osem_subset_b. The reference removes gaps per frame
+ * (mmraux.py:1146-1160, src/scanner_0.cu:94-155) and hands petprj.osem uint16
+ * prompts (prj/src/prj_module.cu:649-651). */
 int nipet_remgaps_u16b(uint16_t *out, const uint32_t *in, const int32_t *aw2ali, int naw,
-                       int nsn, int nbinang, int nrep, int r0, int nb, void *stream);
+                       int nsn, int nbinang, int nrep, int r0, int nb, int out_nb, int out_b0,
+                       void *stream);
 
 /* nipet_remgaps without any device allocation, for callers that cannot
  * allocate (code under HIP graph capture, tight per-frame loops, harnesses
@@ -134,6 +136,13 @@ int nipet_fprj_b(const NipetGeom *g, float *sino, const float *img, const int32_
                  int nprj, int nb, int att, void *stream);
 int nipet_bprj_b(const NipetGeom *g, float *img, const float *sino, const int32_t *subs,
                  int nprj, int nb, void *stream);
+
+/* Sensitivity image of one subset (the images osemone back-projects before the
+ * loop, prj/mmrrec.py:291-298, from the a n sinogram of recon.cu:381-398):
+ * sens = BP_sub(an[subs]) with an the full device sinogram [naw][nsn] read at
+ * the subset's bins (no gathered copy). sens (device image) is overwritten. */
+int nipet_osem_sens(const NipetGeom *g, float *sens, const float *an, const int32_t *subs,
+                    int nprj, void *stream);
 
 /* OSEM corrections, computed once per reconstruction (prj/src/recon.cu:381-398):
  *   rho[i] = (r[i] + s[i]) / (a[i] n[i])  (0 where |a n| < 1e-6),  an[i] = a[i] n[i].
@@ -270,6 +279,19 @@ int nipet_lm_finish(NipetLm *h, const NipetLmOut *out, void *stream);
 int nipet_lm_status(NipetLm *h, int64_t *words_done, unsigned int *bad_words,
                     unsigned int *singles_overflow);
 void nipet_lm_destroy(NipetLm *h);
+
+/* Time-tag index of a device word stream, what getLMinfo finds by scanning the
+ * list-mode file (lm/src/lmaux.cu:17-200: first / last time tag, word positions
+ * that split the stream by time), for words already in device memory.
+ * span (device int64[2]): index of the first time-tag word (n if none) and
+ * one past the index of the last one (0 if none). */
+int nipet_lm_tag_span(const int32_t *words, int64_t n, int64_t *span, void *stream);
+/* first (device int64[nsec + 1]): first[s] = index of the first time-tag word
+ * whose time is at least toff + 1000 s ms (n if none), s = 0 .. nsec; the
+ * words of seconds [t0, t1) are words[first[t0] .. first[t1]). Tags past
+ * toff + 1000 nsec count for nsec, tags before toff for no second. */
+int nipet_lm_second_index(const int32_t *words, int64_t n, int toff, int nsec, int64_t *first,
+                          void *stream);
 
 /* ---- randoms from delayed fan-sums (mmr_lmproc.rand: lm/src/lm_module.cu:404-545,
  * gpu_randoms lm/src/rnd.cu:180-420; Python rand/randoms lm/mmrhist.py:188-221) ----

@@ -77,10 +77,11 @@ def lib():
             'nipet_fprj': ([P, P, P, P, I, I, P], I),
             'nipet_bprj': ([P, P, P, P, I, P], I),
             'nipet_osem_corr': ([P, P, P, P, P, P, I64, P], I),
+            'nipet_osem_sens': ([P, P, P, P, I, P], I),
             'nipet_remgaps': ([P, P, I, P, I, I, I, P], I),
             'nipet_remgaps_noalloc': ([P, P, I, P, I, I, I, P], I),
             'nipet_putgaps': ([P, P, P, I, I, I, P], I),
-            'nipet_remgaps_u16b': ([P, P, P, I, I, I, I, I, I, P], I),
+            'nipet_remgaps_u16b': ([P, P, P, I, I, I, I, I, I, I, I, P], I),
             'nipet_tune_set': ([I, I], I),
             'nipet_psf_conv': ([P, P, P, P, I, P], I),
             'nipet_osem_subset_rm_b': ([P, P, P, P, I, P, I, P, P, I, I, P, P, P, P, P], I),
@@ -96,6 +97,8 @@ def lib():
             'nipet_lm_status': ([P, P, P, P], I),
             'nipet_lm_finish': ([P, ct.POINTER(NipetLmOut), P], I),
             'nipet_lm_destroy': ([P], None),
+            'nipet_lm_tag_span': ([P, I64, P, P], I),
+            'nipet_lm_second_index': ([P, I64, I, I, P, P], I),
             'nipet_rnd_create': ([ct.POINTER(NipetRndDesc), ct.POINTER(P)], I),
             'nipet_rnd_estimate': ([P, P, I, P, P, P], I),
             'nipet_rnd_destroy': ([P], None),

@@ -86,10 +86,14 @@ struct LmDev {
 
 __device__ inline bool is_tag(int32_t w) { return (w >> 29) == -4; }
 
-// last time tag (ms) of every block, -1 if none
+// last time tag (ms) of every block, -1 if none; also clears the chunk's
+// singles-list counter (a kernel write rather than a memset, so that a
+// captured graph replays it)
 __global__ __launch_bounds__(LM_THREADS) void k_lm_tags(const int32_t *words, int64_t n,
-                                                         int32_t *blk_last) {
+                                                         int32_t *blk_last,
+                                                         unsigned int *sgl_count) {
   __shared__ long long best[LM_THREADS / 64];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *sgl_count = 0u;
   const int64_t b0 = (int64_t)blockIdx.x * LM_BLOCK;
   long long key = -1;  // (index << 32) | ms of the last tag seen by this lane
   for (int j = 0; j < LM_PER_LANE; j++) {
@@ -141,9 +145,21 @@ __global__ __launch_bounds__(1024) void k_lm_carry(const int32_t *blk_last, int3
   if (t == 1023 && part[1023] >= 0) carry[0] = part[1023];
 }
 
-__device__ inline uint32_t poisson_n(uint32_t u, const uint32_t *cdf) {
+// N = #{k < LM_NCDF - 1 : u >= cdf[k]}: the inversion count of a
+// non-decreasing uint32 CDF. Counted without branches over the first
+// LM_FASTK entries (held in registers, c8); the remaining entries are read only
+// when some lane of the wave passes all of them (for a mean of 1, a draw
+// reaches N >= 8 with probability 1e-5). A per-lane while loop here compiled
+// into divergent exec-mask bookkeeping that saturated the scalar unit.
+constexpr int LM_FASTK = 8;
+__device__ inline uint32_t poisson_n(uint32_t u, const uint32_t (&c8)[LM_FASTK],
+                                     const uint32_t *cdf) {
   uint32_t N = 0;
-  while (N < LM_NCDF - 1 && u >= cdf[N]) N++;
+#pragma unroll
+  for (int k = 0; k < LM_FASTK; k++) N += u >= c8[k] ? 1u : 0u;
+  if (__any(N == LM_FASTK)) {
+    for (int k = LM_FASTK; k < LM_NCDF - 1; k++) N += u >= cdf[k] ? 1u : 0u;
+  }
   return N;
 }
 
@@ -201,6 +217,9 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
     if (lane == 0) wave_nev[wave] = cnt;
   }
   __syncthreads();
+  uint32_t c8[LM_FASTK];  // the first CDF entries, wave-uniform
+#pragma unroll
+  for (int k = 0; k < LM_FASTK; k++) c8[k] = __builtin_amdgcn_readfirstlane(cdf[k]);
   int32_t c = carry_blk;
   for (int k = 0; k < wave; k++)
     if (wave_last[k] >= 0) c = wave_last[k];
@@ -269,7 +288,7 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
         uint64_t g = (uint64_t)(p.base + gi);
         uint32_t cc[4] = {(uint32_t)g, (uint32_t)(g >> 32), (uint32_t)p.rep0, 0u};
         philox4x32_10(cc, p.k0, p.k1);
-        N = poisson_n(cc[0], cdf);
+        N = poisson_n(cc[0], c8, cdf);
       }
       if (N == 0 || N >= 32) continue;
       const int32_t val = wv & 0x3fffffff;
@@ -354,27 +373,48 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
         }
       }
     } else {
+      // Each lane decodes its own word once (VALU); the loop over the wave's
+      // events then broadcasts the decoded fields with readlane, so the
+      // scalar unit only walks the event mask.
+      int32_t e_addr = 0, e_x1 = 0, e_x2 = -1;
+      if (ev) {
+        const int32_t v = word & 0x3fffffff;
+        const int si = v / NSBINANG, aw = v - si * NSBINANG;
+        const int a = aw / NSBINS, w = aw - a * NSBINS;
+        const int32_t lu = lut[si];
+        const int s11 = lu >> 16, sr = lu & 0xffff;
+        // bin address (< 4084 * 86688 < 2^31)
+        e_addr = p.span == 11 ? s11 * NSBINANG + aw : (p.span == 1 ? v : sr * NSBINANG + aw);
+        if ((word >> 30) & 1) {
+          // prompt: SSRB plane (centre of mass), projection-view offset with
+          // bit 30 set for the second view (a = 126), -1 for none
+          e_x1 = sr;
+          if ((a == 0 || a == 126) && itag < MXNITAG)
+            e_x2 = ((itag >> VTIME) * SEG0 * NSBINS + sr * NSBINS + w) | ((a == 126) << 30);
+        } else {
+          // delayed: the two fan-sum entries
+          const short2 rr = p.rno[si], cr = p.s2cF[a + NSANGLES * w];
+          e_x1 = NCRS * rr.x + cr.x;
+          e_x2 = NCRS * rr.y + cr.y;
+        }
+      }
       unsigned long long evm = __ballot(ev);
       while (evm) {
         const int l = __ffsll((long long)evm) - 1;
         evm &= evm - 1;
         const int32_t wv = __builtin_amdgcn_readlane(word, l);
         const int it = __builtin_amdgcn_readlane(itag, l);
+        const int32_t addr = __builtin_amdgcn_readlane(e_addr, l);
+        const int32_t x1 = __builtin_amdgcn_readlane(e_x1, l);
+        const int32_t x2 = __builtin_amdgcn_readlane(e_x2, l);
         const uint64_t g = (uint64_t)(p.base + w0 + j * 64 + l);
-        const int32_t v = wv & 0x3fffffff;
-        const int si = v / NSBINANG, aw = v - si * NSBINANG;
-        const int a = aw / NSBINS, w = aw - a * NSBINS;
-        const int32_t lu = lut[si];
-        const int s11 = lu >> 16, sr = lu & 0xffff;
-        const int64_t addr = p.span == 11 ? (int64_t)s11 * NSBINANG + aw
-                                          : (p.span == 1 ? (int64_t)v : (int64_t)sr * NSBINANG + aw);
         uint32_t N = 0;
         if (lane < nr) {
           N = 1;
           if (p.btp == 2) {
             uint32_t cc[4] = {(uint32_t)g, (uint32_t)(g >> 32), (uint32_t)(p.rep0 + lane), 0u};
             philox4x32_10(cc, p.k0, p.k1);
-            N = poisson_n(cc[0], cdf);
+            N = poisson_n(cc[0], c8, cdf);
           }
         }
         const bool ok = N > 0 && N < 32;
@@ -382,7 +422,7 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
         const bool inwin = k >= 0 && k < LM_HW;
         const bool prompt = (wv >> 30) & 1;
         if (ok) {
-          atomicAdd((prompt ? p.psino : p.dsino) + addr * nr + lane, N);
+          atomicAdd((prompt ? p.psino : p.dsino) + (int64_t)addr * nr + lane, N);
           uint32_t *hl = prompt ? hcp : hcd;
           uint32_t *hg = prompt ? p.phc : p.dhc;
           if (inwin) atomicAdd(&hl[k * nr + lane], N); else atomicAdd(hg + (int64_t)it * nr + lane, N);
@@ -390,19 +430,16 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
         if (lane == 0 && ok) {
           if (prompt) {
             if (inwin) {
-              atomicAdd(&mzr[k], (unsigned long long)sr);
+              atomicAdd(&mzr[k], (unsigned long long)x1);
               atomicAdd(&mzm[k], (unsigned long long)N);
             } else {
-              atomicAdd(p.zR + it, (unsigned long long)sr);
+              atomicAdd(p.zR + it, (unsigned long long)x1);
               atomicAdd(p.zM + it, (unsigned long long)N);
             }
-            if ((a == 0 || a == 126) && it < MXNITAG)
-              atomicAdd(p.pvs + (int64_t)(it >> VTIME) * SEG0 * NSBINS + sr * NSBINS + w,
-                        N << (8 * (a == 126)));
+            if (x2 >= 0) atomicAdd(p.pvs + (x2 & 0x3fffffff), N << (8 * ((x2 >> 30) & 1)));
           } else {
-            short2 rr = p.rno[si], cr = p.s2cF[a + NSANGLES * w];
-            atomicAdd(p.fan + NCRS * rr.x + cr.x, N);
-            atomicAdd(p.fan + NCRS * rr.y + cr.y, N);
+            atomicAdd(p.fan + x1, N);
+            atomicAdd(p.fan + x2, N);
           }
         }
       }
@@ -598,6 +635,71 @@ static int lm_build(const NipetLmConfig *c, NipetLm *h) {
   return NIPET_OK;
 }
 
+// ---- time-tag index of a device word stream: what getLMinfo finds by scanning
+// the file (first / last time tag, word positions of given times,
+// lm/src/lmaux.cu:17-200), for a stream already in device memory ----
+__global__ void k_lm_tag_span(const int32_t *__restrict__ w, int64_t n,
+                              unsigned long long *span) {
+  // span[0] = smallest index of a time-tag word, span[1] = largest index + 1
+  unsigned long long lo = (unsigned long long)n, hi = 0ull;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (is_tag(w[i])) {
+      lo = lo < (unsigned long long)i ? lo : (unsigned long long)i;
+      hi = (unsigned long long)(i + 1);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long l2 = __shfl_xor(lo, o), h2 = __shfl_xor(hi, o);
+    lo = l2 < lo ? l2 : lo;
+    hi = h2 > hi ? h2 : hi;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (lo < (unsigned long long)n) atomicMin(span, lo);
+    if (hi > 0ull) atomicMax(span + 1, hi);
+  }
+}
+
+// first[b] = smallest index of a tag word whose second (ms - toff) / 1000 is b
+// (tags past the last second count for nsec, tags before toff for none)
+__global__ void k_lm_sec_first(const int32_t *__restrict__ w, int64_t n, int toff, int nsec,
+                               unsigned long long *first) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t x = w[i];
+    if (!is_tag(x)) continue;
+    const int rel = (x & TMSK) - toff;
+    if (rel < 0) continue;
+    const int b = rel / ITIME < nsec ? rel / ITIME : nsec;
+    atomicMin(first + b, (unsigned long long)i);
+  }
+}
+
+// suffix minimum over first[0 .. m): first[s] = min_{b >= s} first[b]; one block
+__global__ __launch_bounds__(1024) void k_lm_suffix_min(unsigned long long *first, int m) {
+  __shared__ unsigned long long t[1024];
+  __shared__ unsigned long long carry;
+  const int tid = threadIdx.x;
+  if (tid == 0) carry = ~0ull;
+  for (int end = m; end > 0; end -= 1024) {
+    const int i = end - 1024 + tid;  // tile [end - 1024, end)
+    __syncthreads();
+    t[tid] = i >= 0 ? first[i] : ~0ull;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      const unsigned long long v = tid + o < 1024 ? t[tid + o] : ~0ull;
+      __syncthreads();
+      if (v < t[tid]) t[tid] = v;
+      __syncthreads();
+    }
+    const unsigned long long c = carry;
+    const unsigned long long r = t[tid] < c ? t[tid] : c;
+    if (i >= 0) first[i] = r;
+    __syncthreads();
+    if (tid == 0) carry = r;  // the tile's first element holds the tile minimum
+  }
+}
+
 extern "C" {
 
 int nipet_lm_create(const NipetLmConfig *c, NipetLm **out) {
@@ -683,9 +785,8 @@ int nipet_lm_chunk(NipetLm *h, const int32_t *words, int64_t n, const NipetLmOut
   unsigned int cap = (unsigned int)std::min<int64_t>(std::max<int64_t>(n / 32, 1 << 20), 1 << 28);
   int rc = grow(h, nblk, cap);
   if (rc) return rc;
-  NIPET_HIP(hipMemsetAsync(h->sgl_count, 0, sizeof(unsigned int), st));
   prof_begin(PK_HIST, st);
-  k_lm_tags<<<(unsigned)nblk, LM_THREADS, 0, st>>>(words, n, h->blk_last);
+  k_lm_tags<<<(unsigned)nblk, LM_THREADS, 0, st>>>(words, n, h->blk_last, h->sgl_count);
   k_lm_carry<<<1, 1024, 0, st>>>(h->blk_last, h->blk_carry, (int)nblk, h->carry);
   LmDev p;
   p.words = words;
@@ -765,6 +866,43 @@ int nipet_lm_status(NipetLm *h, int64_t *words_done, unsigned int *bad_words,
   if (words_done) *words_done = h->base;
   if (bad_words) *bad_words = bad;
   if (singles_overflow) *singles_overflow = cnt > h->sgl_cap ? cnt - h->sgl_cap : 0;
+  return NIPET_OK;
+}
+
+int nipet_lm_tag_span(const int32_t *words, int64_t n, int64_t *span, void *stream) {
+  if (!span || n < 0 || (n > 0 && !words)) {
+    set_error("nipet_lm_tag_span: bad argument");
+    return NIPET_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  unsigned long long *sp = (unsigned long long *)span;
+  k_fill_u64<<<1, 64, 0, st>>>(sp, (unsigned long long)n, 1);
+  k_fill_u64<<<1, 64, 0, st>>>(sp + 1, 0ull, 1);
+  if (n > 0) {
+    const int64_t nb = std::min<int64_t>((n + 255) / 256, 8192);
+    k_lm_tag_span<<<(unsigned)nb, 256, 0, st>>>(words, n, sp);
+  }
+  NIPET_HIP(hipGetLastError());
+  return NIPET_OK;
+}
+
+int nipet_lm_second_index(const int32_t *words, int64_t n, int toff, int nsec, int64_t *first,
+                          void *stream) {
+  if (!first || n < 0 || (n > 0 && !words) || nsec < 0) {
+    set_error("nipet_lm_second_index: bad argument");
+    return NIPET_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  unsigned long long *f = (unsigned long long *)first;
+  const int64_t m = (int64_t)nsec + 1;
+  k_fill_u64<<<(unsigned)std::min<int64_t>((m + 255) / 256, 1024), 256, 0, st>>>(
+      f, (unsigned long long)n, m);
+  if (n > 0) {
+    const int64_t nb = std::min<int64_t>((n + 255) / 256, 8192);
+    k_lm_sec_first<<<(unsigned)nb, 256, 0, st>>>(words, n, toff, nsec, f);
+  }
+  k_lm_suffix_min<<<1, 1024, 0, st>>>(f, (int)m);
+  NIPET_HIP(hipGetLastError());
   return NIPET_OK;
 }
 

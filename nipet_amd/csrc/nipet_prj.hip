@@ -703,7 +703,8 @@ constexpr int bp_spw() { return NB == 1 ? 8 : (NB == 2 ? 4 : 6); }  // measured 
 template <int NB>
 __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g, float *__restrict__ img,
                                                      const float *__restrict__ sino,
-                                                     const int32_t *__restrict__ subs, Batch bt) {
+                                                     const int32_t *__restrict__ subs, Batch bt,
+                                                     int by_bin) {
   constexpr int SPW = bp_spw<NB>();
   static_assert(SPW <= 8, "the run word holds the boundary bits and steps of 8 segments");
   constexpr int NWAVE = BP_THREADS / 64;
@@ -715,7 +716,9 @@ __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g
   const int aw = subs ? subs[row] : row;
   if (aw < 0 || aw >= g.naw) return;
   const int nsn = g.nsn, nbt = bt.nbt, b0 = bt.b0;
-  const float *in = sino + (size_t)row * nsn * nbt + b0;
+  // sinogram rows in subset order, or (by_bin) the full [naw][nsn] sinogram
+  // read at the bin's own row
+  const float *in = sino + (size_t)(by_bin ? aw : row) * nsn * nbt + b0;
   const int beg = g.off[aw], nseg = g.off[aw + 1] - beg;
   const double atn = g.atn[aw];
 
@@ -969,10 +972,11 @@ __global__ void k_osem_update_pair(float *img, float *sens, const float2 *b2,
 }
 
 // Siemens histogram with replicates innermost, in[(s * nbinang + ali) * nrep + r]
-// (uint32 counts) -> device prompts out[(aw * nsn + s) * nb + b] (uint16,
-// saturating at 65535) for replicates r = r0 + b, b < nb.
+// (uint32 counts) -> device prompts out[(aw * nsn + s) * out_nb + out_b0 + b]
+// (uint16, saturating at 65535) for replicates r = r0 + b, b < nb.
 __global__ void k_remgaps_u16b(uint16_t *out, const uint32_t *in, const int32_t *aw2ali, int naw,
-                               int nsn, int nbinang, int nrep, int r0, int nb) {
+                               int nsn, int nbinang, int nrep, int r0, int nb, int out_nb,
+                               int out_b0) {
   const int64_t n = (int64_t)naw * nsn * nb;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -980,7 +984,7 @@ __global__ void k_remgaps_u16b(uint16_t *out, const uint32_t *in, const int32_t 
     const int b = (int)(i - t * nb);
     const int aw = (int)(t / nsn), s = (int)(t - (int64_t)aw * nsn);
     const uint32_t v = in[((int64_t)s * nbinang + aw2ali[aw]) * nrep + r0 + b];
-    out[i] = (uint16_t)(v > 65535u ? 65535u : v);
+    out[t * out_nb + out_b0 + b] = (uint16_t)(v > 65535u ? 65535u : v);
   }
 }
 
@@ -1088,19 +1092,19 @@ static void launch_fprj(const Geom *g, float *sino, const float *img, const int3
 }
 
 static void launch_bprj(const Geom *g, float *img, const float *sino, const int32_t *subs,
-                        int nprj, int nb, hipStream_t st) {
+                        int nprj, int nb, hipStream_t st, int by_bin = 0) {
   const int wmax = g_width_bp.load();
   for (int b0 = 0; b0 < nb;) {
     const int rem = nb - b0;
     const Batch bt{nb, b0};
     if (rem >= 4 && wmax >= 4) {
-      k_bprj<4><<<nprj, BP_THREADS, 0, st>>>(view(g), img, sino, subs, bt);
+      k_bprj<4><<<nprj, BP_THREADS, 0, st>>>(view(g), img, sino, subs, bt, by_bin);
       b0 += 4;
     } else if (rem >= 2 && wmax >= 2) {
-      k_bprj<2><<<nprj, BP_THREADS, 0, st>>>(view(g), img, sino, subs, bt);
+      k_bprj<2><<<nprj, BP_THREADS, 0, st>>>(view(g), img, sino, subs, bt, by_bin);
       b0 += 2;
     } else {
-      k_bprj<1><<<nprj, BP_THREADS, 0, st>>>(view(g), img, sino, subs, bt);
+      k_bprj<1><<<nprj, BP_THREADS, 0, st>>>(view(g), img, sino, subs, bt, by_bin);
       b0 += 1;
     }
   }
@@ -1185,6 +1189,26 @@ int nipet_bprj_b(const NipetGeom *g, float *img, const float *sino, const int32_
 int nipet_bprj(const NipetGeom *g, float *img, const float *sino, const int32_t *subs, int nprj,
                void *stream) {
   return nipet_bprj_b(g, img, sino, subs, nprj, 1, stream);
+}
+
+int nipet_osem_sens(const NipetGeom *g, float *sens, const float *an, const int32_t *subs,
+                    int nprj, void *stream) {
+  int rc = check_geom(g, "nipet_osem_sens");
+  if (rc) return rc;
+  if ((rc = check_nprj(g, subs, nprj, 1, "nipet_osem_sens"))) return rc;
+  if (!sens || !an || !subs) {
+    set_error("nipet_osem_sens: NULL buffer");
+    return NIPET_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  prof_begin(PK_FILL, st);
+  k_fill<<<ew_blocks(NVOX), 256, 0, st>>>(sens, 0.f, NVOX);
+  prof_end(PK_FILL, st);
+  prof_begin(PK_BPRJ, st);
+  launch_bprj(g, sens, an, subs, nprj, 1, st, 1);
+  prof_end(PK_BPRJ, st);
+  NIPET_HIP(hipGetLastError());
+  return NIPET_OK;
 }
 
 int nipet_remgaps(float *out, const void *in, int in_dtype, const int32_t *aw2ali, int naw,
@@ -1274,16 +1298,18 @@ int nipet_remgaps_noalloc(float *out, const void *in, int in_dtype, const int32_
 }
 
 int nipet_remgaps_u16b(uint16_t *out, const uint32_t *in, const int32_t *aw2ali, int naw,
-                       int nsn, int nbinang, int nrep, int r0, int nb, void *stream) {
+                       int nsn, int nbinang, int nrep, int r0, int nb, int out_nb, int out_b0,
+                       void *stream) {
   if (!out || !in || !aw2ali || naw <= 0 || nsn <= 0 || nbinang <= 0 || nrep < 1 || r0 < 0 ||
-      nb < 1 || r0 + nb > nrep) {
-    set_error("nipet_remgaps_u16b: bad arguments (naw=%d nsn=%d nrep=%d r0=%d nb=%d)", naw, nsn,
-              nrep, r0, nb);
+      nb < 1 || r0 + nb > nrep || out_b0 < 0 || out_b0 + nb > out_nb) {
+    set_error("nipet_remgaps_u16b: bad arguments (naw=%d nsn=%d nrep=%d r0=%d nb=%d out_nb=%d "
+              "out_b0=%d)", naw, nsn, nrep, r0, nb, out_nb, out_b0);
     return NIPET_EINVAL;
   }
   const int64_t n = (int64_t)naw * nsn * nb;
   k_remgaps_u16b<<<ew_blocks(n), 256, 0, (hipStream_t)stream>>>(out, in, aw2ali, naw, nsn,
-                                                               nbinang, nrep, r0, nb);
+                                                               nbinang, nrep, r0, nb, out_nb,
+                                                               out_b0);
   NIPET_HIP(hipGetLastError());
   return NIPET_OK;
 }
@@ -1449,7 +1475,7 @@ int nipet_osem_subset_sens(const NipetGeom *g, float *img, const uint16_t *promp
   // both images through one traversal of the rays (independent of the
   // batch autotuner's width)
   prof_begin(PK_BPRJ_PAIR, st);
-  k_bprj<2><<<nprj, BP_THREADS, 0, st>>>(view(g), work_img2, work_sino2, subs, Batch{2, 0});
+  k_bprj<2><<<nprj, BP_THREADS, 0, st>>>(view(g), work_img2, work_sino2, subs, Batch{2, 0}, 0);
   prof_end(PK_BPRJ_PAIR, st);
   prof_begin(PK_OSEM_UPDATE, st);
   k_osem_update_pair<<<ew_blocks(NVOX), 256, 0, st>>>(img, sens, (const float2 *)work_img2, mask,

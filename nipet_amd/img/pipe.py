@@ -245,7 +245,7 @@ def _batched(items, datain, scanner_params, itr, fwhm, psf, mask_radius, decay_r
     states = prepare_groups(scanner_params, groups, d0['asng'], d0['nsng'], rs, None, psf=krnl,
                             mask_radius=mask_radius)
     autotune(states[0])
-    imgs = OsemGraph(states, itr).run().clone()
+    imgs = OsemGraph(states, itr).run()
     out = {}
     for f, (ifrm, hst, d) in enumerate(items):
         img = imgs[..., f].contiguous()

@@ -28,8 +28,8 @@ from .. import _lib
 
 log = logging.getLogger(__name__)
 
-__all__ = ['lminfo', 'mmrhist', 'hist', 'LmHistogram', 'hist_words', 'Randoms', 'randoms',
-           'rand', 'dynamic_timings']
+__all__ = ['lminfo', 'mmrhist', 'hist', 'LmHistogram', 'hist_words', 'tag_range',
+           'second_index', 'Randoms', 'randoms', 'rand', 'dynamic_timings']
 
 CHUNK_WORDS = 1 << 26      # 256 MB of words per transfer
 
@@ -149,14 +149,30 @@ def tag_range(words):
     on the device; else NumPy)."""
     torch = _torch()
     if isinstance(words, torch.Tensor) and words.is_cuda:
-        idx = torch.nonzero((words >> 29) == -4).squeeze(1)
-        if idx.numel() == 0:
+        w = words.contiguous()
+        span = torch.empty(2, dtype=torch.int64, device='cuda')
+        _lib.check(_lib.lib().nipet_lm_tag_span(_lib.ptr(w), w.numel(), _lib.ptr(span),
+                                                _lib.stream_handle()), 'nipet_lm_tag_span')
+        i0, i1 = (int(v) for v in span.cpu().numpy())
+        if i1 == 0:
             raise ValueError('no time tags in the list-mode data')
-        ends = words[idx[[0, -1]]].cpu().numpy()
-        return int(ends[0] & 0x1fffffff), int(ends[1] & 0x1fffffff)
+        return int(w[i0]) & 0x1fffffff, int(w[i1 - 1]) & 0x1fffffff
     w = words.numpy() if isinstance(words, torch.Tensor) else np.asarray(words)
     (tf, _), (tl, _) = _tag_positions(w)
     return tf, tl
+
+
+def second_index(words, toff, nsec):
+    """Host int64 array [nsec + 1]: word index of the first time tag of each
+    second s = 0 .. nsec after toff in a device word array (the number of
+    words where none follows), found on the device."""
+    torch = _torch()
+    w = words.contiguous()
+    first = torch.empty(nsec + 1, dtype=torch.int64, device='cuda')
+    _lib.check(_lib.lib().nipet_lm_second_index(_lib.ptr(w), w.numel(), int(toff), int(nsec),
+                                                _lib.ptr(first), _lib.stream_handle()),
+               'nipet_lm_second_index')
+    return first.cpu().numpy()
 
 
 def hist_words(words, scanner_params, t0=0, t1=0, btp=None, seed=0, nrep=1, span=None, rep0=0):

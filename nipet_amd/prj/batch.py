@@ -179,9 +179,21 @@ class OsemGraph:
         return self.imgs
 
     def run(self, img0=None):
-        """``run_groups`` joined into one [320, 320, 127, nb] tensor."""
-        imgs = self.run_groups(img0)
-        return imgs[0] if len(imgs) == 1 else _torch().cat(imgs, dim=3)
+        """``run_groups`` joined into one new [320, 320, 127, nb] tensor."""
+        return _join(self.run_groups(img0))
+
+
+def _join(imgs):
+    """Per-group images [320, 320, 127, g] copied into one new
+    [320, 320, 127, sum g] tensor."""
+    torch = _torch()
+    nb = sum(x.shape[3] for x in imgs)
+    out = torch.empty((320, 320, 127, nb), dtype=torch.float32, device='cuda')
+    off = 0
+    for x in imgs:
+        out[..., off:off + x.shape[3]].copy_(x)
+        off += x.shape[3]
+    return out
 
 
 def bootstrap_prompts(words, scanner_params, nrep, seed, rep0=0, t0=0, t1=0, hist=None,
@@ -199,15 +211,21 @@ def bootstrap_prompts(words, scanner_params, nrep, seed, rep0=0, t0=0, t1=0, his
             for r0 in range(0, h.nrep, group)], h
 
 
-def _remgaps_b(h, scanner_params, nrep, r0, nb):
+def _remgaps_b(h, scanner_params, nrep, r0, nb, out=None, out_b0=0):
+    """uint16 prompts (Naw, nsn, nb) of replicates r0 .. r0+nb-1 of a histogram
+    session, or written into columns out_b0 .. out_b0+nb-1 of ``out``."""
     torch = _torch()
     ali = torch.from_numpy(np.asarray(scanner_params['txLUT']['aw2ali'], np.int32)).cuda()
     naw = ali.numel()
     nsn = h.nsinos
-    out = torch.empty((naw, nsn, nb), dtype=torch.uint16, device='cuda')
+    if out is None:
+        out = torch.empty((naw, nsn, nb), dtype=torch.uint16, device='cuda')
+    if out.shape[:2] != (naw, nsn) or not out.is_contiguous():
+        raise ValueError(f'prompts output must be a contiguous ({naw}, {nsn}, n) tensor')
     _lib.check(_lib.lib().nipet_remgaps_u16b(_lib.ptr(out), _lib.ptr(h.out['psino']),
                                              _lib.ptr(ali), naw, nsn, 86688, nrep, r0, nb,
-                                             _lib.stream_handle()), 'nipet_remgaps_u16b')
+                                             out.shape[2], out_b0, _lib.stream_handle()),
+               'nipet_remgaps_u16b')
     return out
 
 
@@ -216,26 +234,27 @@ def frame_prompts(words, scanner_params, frames):
     (seconds from the first tag) of a device word array, one histogram session
     per frame over the words of its window. Returns (prompts, [delayed totals])."""
     torch = _torch()
-    from ..lm.mmrhist import LmHistogram, tag_range
+    from ..lm.mmrhist import LmHistogram, second_index, tag_range
     tf, tl = tag_range(words)
     nitag = ((tl - tf) + 999) // 1000
-    tags = torch.nonzero((words >> 29) == -4).squeeze(1)
-    tms = ((words[tags] & 0x1fffffff) - tf).to(torch.int64)
-    out = []
-    dtot = []
     for t0, t1 in frames:
         if not (0 <= t0 < t1 <= nitag):
             raise ValueError(f'frame ({t0}, {t1}) outside 0..{nitag} s')
+    # word index of the first tag of every second (lm/src/lmaux.cu:17-200)
+    first = second_index(words, tf, nitag)
+    naw = len(scanner_params['txLUT']['aw2ali'])
+    out = torch.empty((naw, scanner_params['Cnt']['NSN11'], len(frames)), dtype=torch.uint16,
+                      device='cuda')
+    dtot = []
+    for f, (t0, t1) in enumerate(frames):
         # words from the first tag of second t0 to the first tag of second t1
-        i0 = int(tags[int(torch.searchsorted(tms, t0 * 1000))])
-        k1 = int(torch.searchsorted(tms, t1 * 1000))
-        i1 = int(tags[k1]) if k1 < tags.numel() else words.numel()
+        i0, i1 = int(first[t0]), int(first[t1])
         h = LmHistogram(scanner_params, tf, nitag, t0, t1, btp=0)
         h.add(words[i0:i1])
-        out.append(_remgaps_b(h, scanner_params, 1, 0, 1))
-        dtot.append(int(h.out['dsino'].sum(dtype=torch.int64)))
+        _remgaps_b(h, scanner_params, 1, 0, 1, out=out, out_b0=f)
+        dtot.append(int(h.out['dhc'].cpu().numpy().astype(np.int64).sum()))
         del h
-    return torch.cat(out, dim=2).contiguous(), dtot
+    return out, dtot
 
 
 def bootstrap_osem(words, scanner_params, asng, nsng, rsng=None, ssng=None, nrep=64, seed=1,
@@ -267,14 +286,14 @@ def bootstrap_osem(words, scanner_params, asng, nsng, rsng=None, ssng=None, nrep
     if graph:
         g = OsemGraph(states, itr)
         mark('capture_s')
-        img = g.run().clone()
+        img = g.run()
     else:
         imgs = [torch.ones((320, 320, 127, x.nb), dtype=torch.float32, device='cuda')
                 for x in states]
         for _ in range(itr):
             for x, im in zip(states, imgs):
                 osem_iteration_batch(x, im)
-        img = torch.cat(imgs, dim=3)
+        img = _join(imgs)
     mark('osem_s')
     info['sens'] = st.sens
     info['timing'] = tm
@@ -294,14 +313,14 @@ def frames_osem(words, scanner_params, frames, asng, nsng, rho_frames, itr=4, tu
                             psf=psf)
     info = {'tune': autotune(states[0]) if tune else None, 'delayed': dtot}
     if graph:
-        img = OsemGraph(states, itr).run().clone()
+        img = OsemGraph(states, itr).run()
     else:
         imgs = [torch.ones((320, 320, 127, x.nb), dtype=torch.float32, device='cuda')
                 for x in states]
         for _ in range(itr):
             for x, im in zip(states, imgs):
                 osem_iteration_batch(x, im)
-        img = torch.cat(imgs, dim=3)
+        img = _join(imgs)
     return img, info
 
 

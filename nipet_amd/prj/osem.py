@@ -141,13 +141,14 @@ class OsemState:
 
 
 def sensitivity(geom, an, subs, out=None):
-    """BP_subset(a*n) (``prj/mmrrec.py:291-298``) for one subset (device tensors)."""
+    """BP_subset(a*n) (``prj/mmrrec.py:291-298``) for one subset (device tensors:
+    ``an`` the full (Naw, nsn) a*n sinogram, read at the subset's bins)."""
     torch = _torch()
-    rows = an.index_select(0, subs.long())
     if out is None:
         out = torch.empty((320, 320, 127), dtype=torch.float32, device='cuda')
-    _lib.check(_lib.lib().nipet_bprj(geom.handle, _lib.ptr(out), _lib.ptr(rows), _lib.ptr(subs),
-                                     len(subs), _lib.stream_handle()), 'nipet_bprj')
+    _lib.check(_lib.lib().nipet_osem_sens(geom.handle, _lib.ptr(out), _lib.ptr(an),
+                                          _lib.ptr(subs), len(subs), _lib.stream_handle()),
+               'nipet_osem_sens')
     return out
 
 
@@ -217,7 +218,9 @@ def osem_prepare(scanner_params, psng, asng, nsng, rsng=None, ssng=None, mask_ra
         sens = torch.empty((len(subs), 320, 320, 127), dtype=torch.float32, device='cuda')
         pending = range(len(subs))
     else:
-        sens = torch.stack([sensitivity(geom, an, x) for x in subs])
+        sens = torch.empty((len(subs), 320, 320, 127), dtype=torch.float32, device='cuda')
+        for k, x in enumerate(subs):
+            sensitivity(geom, an, x, out=sens[k])
         pending = None
     if krnl is not None:
         tmp = torch.empty((320, 320, 127), dtype=torch.float32, device='cuda')
@@ -355,7 +358,9 @@ def osem_prepare_batch(scanner_params, prompts, asng, nsng, rsng=None, ssng=None
     subs = [_longest_first(geom, x) for x in subsets]
     krnl = _psf_kernel(psf)
     if sens is None:
-        sens = torch.stack([sensitivity(geom, an, x) for x in subs])
+        sens = torch.empty((len(subs), 320, 320, 127), dtype=torch.float32, device='cuda')
+        for k, x in enumerate(subs):
+            sensitivity(geom, an, x, out=sens[k])
         if krnl is not None:
             tmp = torch.empty((320, 320, 127), dtype=torch.float32, device='cuda')
             for k in range(len(subs)):

@@ -156,6 +156,30 @@ def test_mmrhist_sinogram_cache(torch, params, stream, tmp_path):
         nip.mmrhist({'sinos': cache, 'lm_bf': f}, params, t0=0, t1=5, use_stored=True)
 
 
+def test_time_index_on_device(torch, stream):
+    """First / last time tag and the first tag word of every second, found on
+    the device, against a NumPy scan of the same words; a stream without tags
+    and seconds past the last tag give the word count."""
+    from nipet_amd.lm.mmrhist import second_index, tag_range
+    w = torch.from_numpy(stream).cuda()
+    t = np.nonzero((stream >> 29) == -4)[0]
+    tf, tl = int(stream[t[0]] & 0x1fffffff), int(stream[t[-1]] & 0x1fffffff)
+    assert tag_range(w) == (tf, tl)
+    nsec = ((tl - tf) + 999) // 1000
+    got = second_index(w, tf, nsec + 2)
+    ms = (stream[t] & 0x1fffffff).astype(np.int64) - tf
+    want = [int(t[np.argmax(ms >= 1000 * s)]) if np.any(ms >= 1000 * s) else len(stream)
+            for s in range(nsec + 3)]
+    assert got.tolist() == want
+    assert got[-1] == len(stream)
+    # a slice without tags
+    events = torch.from_numpy(stream[1:200].copy()).cuda()
+    assert ((stream[1:200] >> 29) == -4).sum() == 0
+    with pytest.raises(ValueError):
+        tag_range(events)
+    assert second_index(events, tf, 3).tolist() == [199] * 4
+
+
 def test_rep0_selects_replicate(torch, params, stream):
     """A session with nrep = 1, rep0 = 2 histograms exactly replicate 2 of an
     nrep = 3 session (oracle), so replicate sets can be split across sessions."""
@@ -181,13 +205,25 @@ def test_remgaps_u16b(torch, params, stream):
     out = torch.empty((naw, 837, 3), dtype=torch.uint16, device='cuda')
     ali_d = torch.from_numpy(ali).cuda()
     _lib.check(_lib.lib().nipet_remgaps_u16b(out.data_ptr(), h.out['psino'].data_ptr(),
-                                             ali_d.data_ptr(), naw, 837, 86688, 5, 1, 3,
+                                             ali_d.data_ptr(), naw, 837, 86688, 5, 1, 3, 3, 0,
                                              _lib.stream_handle()), 'remgaps_u16b')
     torch.cuda.synchronize()
     ref = g['psino'].reshape(837, 86688, 5)[:, ali, 1:4].transpose(1, 0, 2)
     assert np.array_equal(out.cpu().numpy().astype(np.int64), np.minimum(ref, 65535))
+    # replicate 4 alone into column 2 of a 3-wide output, columns 0-1 untouched
+    out.fill_(7)
+    _lib.check(_lib.lib().nipet_remgaps_u16b(out.data_ptr(), h.out['psino'].data_ptr(),
+                                             ali_d.data_ptr(), naw, 837, 86688, 5, 4, 1, 3, 2,
+                                             _lib.stream_handle()), 'remgaps_u16b')
+    got = out.cpu().numpy().astype(np.int64)
+    assert np.array_equal(got[..., 2], np.minimum(g['psino'].reshape(837, 86688, 5)[:, ali, 4].T,
+                                                  65535))
+    assert np.all(got[..., :2] == 7)
     bad = _lib.lib().nipet_remgaps_u16b(out.data_ptr(), h.out['psino'].data_ptr(),
-                                        ali_d.data_ptr(), naw, 837, 86688, 5, 3, 3, None)
+                                        ali_d.data_ptr(), naw, 837, 86688, 5, 3, 3, 3, 0, None)
+    assert bad != 0
+    bad = _lib.lib().nipet_remgaps_u16b(out.data_ptr(), h.out['psino'].data_ptr(),
+                                        ali_d.data_ptr(), naw, 837, 86688, 5, 0, 2, 3, 2, None)
     assert bad != 0
 
 
