@@ -308,7 +308,10 @@ def batched(params, d, words, rs, nrep):
             'plain_s_per_replicate': round(tr, 4), 'plain_s_per_frame': round(tf, 4),
             'plain_checked': len(rels), 'max_rel_l2_vs_plain': max(rels.values()),
             'rel_l2_vs_plain': rels,
-            'widths': {'fprj': info['tune']['fprj'], 'bprj': info['tune']['bprj']},
+            'autotuned': {'fprj_width': info['tune']['fprj'], 'bprj_width': info['tune']['bprj'],
+                          'bprj_run_by_width': info['tune']['bprj_run'],
+                          'ms_per_image': {'/'.join(str(x) for x in k): round(v, 4)
+                                           for k, v in info['tune']['ms'].items()}},
             'batch_breakdown_s': info['timing'],
             'timing': 'wall clock incl. histogramming, gap removal, corrections, sensitivity, '
                       'autotune, graph capture; plain = sequential estimate from the timed '

@@ -389,6 +389,13 @@ int nipet_prof_enable(int on);
  * projectors take one image). */
 int nipet_tune_set(int kernel, int width);
 int nipet_tune_get(int kernel, int *width);
+/* Segments per wave run of the back projector at sub-batch width `width` (1, 2
+ * or 4): one of the two run lengths compiled for that width (1: 8 or 6;
+ * 2: 4 or 6; 4: 6 or 4). Process-wide; set by the autotuner. nipet_tune_get_run
+ * also returns the other compiled length in *alt (may be NULL). No counterpart
+ * in the reference (its ray blocks are fixed at compile time). */
+int nipet_tune_set_run(int width, int spw);
+int nipet_tune_get_run(int width, int *spw, int *alt);
 int nipet_prof_reset(void);
 int nipet_prof_read(int kernel, double *total_ms, int64_t *count);
 

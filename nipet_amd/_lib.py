@@ -86,6 +86,8 @@ def lib():
             'nipet_psf_conv': ([P, P, P, P, I, P], I),
             'nipet_osem_subset_rm_b': ([P, P, P, P, I, P, I, P, P, I, I, P, P, P, P, P], I),
             'nipet_tune_get': ([I, P], I),
+            'nipet_tune_set_run': ([I, I], I),
+            'nipet_tune_get_run': ([I, P, P], I),
             'nipet_osem_subset': ([P, P, P, P, P, P, P, I, P, P, P], I),
             'nipet_osem_subset_sens': ([P, P, P, P, P, P, P, P, I, P, P, P, P], I),
             'nipet_osem_iter': ([P, P, P, P, P, P, P, P, I, I, P, P, P, P], I),

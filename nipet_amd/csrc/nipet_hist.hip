@@ -47,6 +47,19 @@ constexpr int LM_MAXREP = 64;
 constexpr int LM_NCDF = 40;
 constexpr int NSBINANG = 86688, NSBINS = 344, NSANGLES = 252, NCRS = 504, NBUCKTS = 224;
 constexpr int NSN1 = 4084, SEG0 = 127, MXNITAG = 5400, VTIME = 2, ITIME = 1000;
+constexpr int64_t NSN1_BINS_MAX = (int64_t)NSN1 * NSBINANG;
+// LDS-privatised sinogram counts (one replicate): bins are counted per bucket
+// of LM_BBINS consecutive bins, whose prompt and delayed counters fit in LDS
+constexpr int LM_BSHIFT = 14;
+constexpr int LM_BBINS = 1 << LM_BSHIFT;
+// Events reach their bucket in two partition passes whose writes are runs:
+// each block routes its events to coarse buckets of 2^LM_CSHIFT bins (about a
+// hundred events per coarse bucket and block), then tiles of LM_TILE staged
+// events are sorted in LDS by bucket. LM_MAXCB covers span 1.
+constexpr int LM_CSHIFT = 20;
+constexpr int LM_FPC = 1 << (LM_CSHIFT - LM_BSHIFT);  // buckets per coarse bucket
+constexpr int LM_MAXCB = (NSN1_BINS_MAX + (1 << LM_CSHIFT) - 1) >> LM_CSHIFT;
+constexpr int LM_TILE = 8192;
 constexpr int32_t TMSK = 0x1fffffff;
 
 __host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
@@ -82,9 +95,58 @@ struct LmDev {
   unsigned int *sgl_count;
   unsigned int sgl_cap;
   unsigned int *bad;
+  // LDS-privatised sinogram counts: events staged per coarse bucket
+  uint32_t *ccur;   // [ncb] next free slot of each coarse bucket
+  uint32_t *stage;  // (bin within coarse bucket) | prompt << LM_CSHIFT | N << (LM_CSHIFT + 1)
+  int64_t stage_cap;
+  int ncb;
 };
 
 __device__ inline bool is_tag(int32_t w) { return (w >> 29) == -4; }
+
+// Output bin of an event word's span-1 address (span 11, 1 or SSRB).
+__device__ inline int64_t lm_bin(int span, int32_t val, const int32_t *lut) {
+  const int si = val / NSBINANG, aw = val - si * NSBINANG;
+  const int32_t lu = lut[si];
+  return span == 11 ? (int64_t)(lu >> 16) * NSBINANG + aw
+                    : (span == 1 ? (int64_t)val : (int64_t)(lu & 0xffff) * NSBINANG + aw);
+}
+
+// The LM_PER_LANE words of one lane of a block's wave (word w0 + 64 j + lane;
+// 0 past the end); returns the last time tag (ms) among the wave's words, -1
+// if none.
+__device__ inline int32_t lm_load(const LmDev &p, int64_t w0, int lane,
+                                  int32_t (&wd)[LM_PER_LANE]) {
+#pragma unroll
+  for (int j = 0; j < LM_PER_LANE; j++) {
+    const int64_t i = w0 + j * 64 + lane;
+    wd[j] = i < p.n ? p.words[i] : 0;
+  }
+  int32_t wl = -1;
+#pragma unroll
+  for (int j = 0; j < LM_PER_LANE; j++) {
+    const unsigned long long m = __ballot(is_tag(wd[j]));
+    if (m) wl = __builtin_amdgcn_readlane(wd[j] & TMSK, 63 - __clzll(m));
+  }
+  return wl;
+}
+
+// Time tag (ms) in effect at this lane's word of a 64-word group: the most
+// recent tag at or before it, c (wave-uniform) when the group holds none
+// before it; c then moves to the group's last tag. le = lanes 0 .. lane.
+__device__ inline int32_t lm_tag_at(int32_t word, int32_t &c, unsigned long long le) {
+  int32_t ms = c;
+  const unsigned long long m = __ballot(is_tag(word));
+  if (m) {
+    const unsigned long long ml = m & le;
+    const int32_t tv = word & TMSK;
+    const int src = ml ? 63 - __clzll(ml) : (int)__lane_id();
+    const int32_t got = __shfl(tv, src);
+    if (ml) ms = got;
+    c = __builtin_amdgcn_readlane(tv, 63 - __clzll(m));
+  }
+  return ms;
+}
 
 // last time tag (ms) of every block, -1 if none; also clears the chunk's
 // singles-list counter (a kernel write rather than a memset, so that a
@@ -168,8 +230,14 @@ __device__ inline uint32_t poisson_n(uint32_t u, const uint32_t (&c8)[LM_FASTK],
 // NP: non-parametric bootstrap (btp = 1): the block's event words are
 // compacted in word order into LDS (the resampling patch) and every counted
 // event is replaced by a draw from them.
-template <bool REP, bool NP>
+// BIN (one replicate, no resampling): the prompt / delayed sinogram counts are
+// not added here. The block counts its events per coarse bucket, reserves one
+// run of slots per coarse bucket, and stages every counted event in its run
+// (the coarse slot ranges were counted by k_lm_bin_count); k_lm_bin_part and
+// k_lm_bin_apply then count them in LDS.
+template <bool REP, bool NP, bool BIN = false>
 __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
+  static_assert(!BIN || (!REP && !NP), "binned counts serve one replicate without resampling");
   __shared__ int32_t lut[NSN1];
   __shared__ int32_t evw[NP ? LM_BLOCK : 1];
   __shared__ int32_t wave_nev[LM_WAVES];
@@ -178,9 +246,13 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
   __shared__ uint32_t hcd[LM_HW * (REP ? LM_MAXREP : 1)];
   __shared__ unsigned long long mzr[LM_HW], mzm[LM_HW];
   __shared__ int32_t wave_last[LM_WAVES];
+  __shared__ uint32_t crank[BIN ? LM_MAXCB : 1], cbase[BIN ? LM_MAXCB : 1];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nr = REP ? p.nrep : 1;
   for (int i = tid; i < NSN1; i += LM_THREADS) lut[i] = p.lut[i];
+  if constexpr (BIN) {
+    for (int i = tid; i < p.ncb; i += LM_THREADS) crank[i] = 0;
+  }
   if (tid < LM_NCDF) cdf[tid] = p.cdf[tid];
   for (int i = tid; i < LM_HW * nr; i += LM_THREADS) {
     hcp[i] = 0;
@@ -196,18 +268,8 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
   const int itag0 = carry_blk < 0 ? 0 : (carry_blk - p.toff) / ITIME;  // window base
 
   int32_t wd[LM_PER_LANE];
-#pragma unroll
-  for (int j = 0; j < LM_PER_LANE; j++) {
-    int64_t i = w0 + j * 64 + lane;
-    wd[j] = i < p.n ? p.words[i] : 0;
-  }
   // pass 1: last tag of each wave's words
-  int32_t wl = -1;
-#pragma unroll
-  for (int j = 0; j < LM_PER_LANE; j++) {
-    unsigned long long m = __ballot(is_tag(wd[j]));
-    if (m) wl = __builtin_amdgcn_readlane(wd[j] & TMSK, 63 - __clzll(m));
-  }
+  const int32_t wl = lm_load(p, w0, lane, wd);
   if (lane == 0) wave_last[wave] = wl;
   int nev = 0;  // events of the block (NP)
   if constexpr (NP) {
@@ -241,19 +303,30 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
   }
 
   const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1);
+  if constexpr (BIN) {
+    // the block's counted events per coarse bucket (the same walk and test as
+    // below), then one run of slots per coarse bucket
+    int32_t c1 = c;
+    for (int j = 0; j < LM_PER_LANE; j++) {
+      const int32_t word = wd[j];
+      const int32_t ms = lm_tag_at(word, c1, le);
+      const int itag = (ms < 0 ? 0 : ms - p.toff) / ITIME;
+      if (word > 0 && itag >= p.tstart && itag < p.tstop &&
+          (word & 0x3fffffff) < NSN1 * NSBINANG)
+        atomicAdd(&crank[lm_bin(p.span, word & 0x3fffffff, lut) >> LM_CSHIFT], 1u);
+    }
+    __syncthreads();
+    for (int i = tid; i < p.ncb; i += LM_THREADS) {
+      const uint32_t nc = crank[i];
+      cbase[i] = nc ? atomicAdd(p.ccur + i, nc) : 0u;
+      crank[i] = 0;
+    }
+    __syncthreads();
+  }
   for (int j = 0; j < LM_PER_LANE; j++) {
     const int32_t word = wd[j];
     const int64_t gi = w0 + j * 64 + lane;
-    int32_t ms = c;
-    unsigned long long m = __ballot(is_tag(word));
-    if (m) {
-      unsigned long long ml = m & le;
-      int32_t tv = word & TMSK;
-      int src = ml ? 63 - __clzll(ml) : lane;
-      int32_t got = __shfl(tv, src);
-      if (ml) ms = got;
-      c = __builtin_amdgcn_readlane(tv, 63 - __clzll(m));
-    }
+    const int32_t ms = lm_tag_at(word, c, le);
     const int rel = ms < 0 ? 0 : ms - p.toff;
     const int itag = rel / ITIME;
     const bool in_frame = itag >= p.tstart && itag < p.tstop;
@@ -290,7 +363,10 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
         philox4x32_10(cc, p.k0, p.k1);
         N = poisson_n(cc[0], c8, cdf);
       }
-      if (N == 0 || N >= 32) continue;
+      const bool okN = N > 0 && N < 32;
+      if constexpr (!BIN) {
+        if (!okN) continue;
+      }
       const int32_t val = wv & 0x3fffffff;
       const int si = val / NSBINANG, aw = val - si * NSBINANG;
       const int a = aw / NSBINS, w = aw - a * NSBINS;
@@ -298,10 +374,20 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
       const int s11 = lu >> 16, sr = lu & 0xffff;
       const int64_t addr = p.span == 11 ? (int64_t)s11 * NSBINANG + aw
                                         : (p.span == 1 ? (int64_t)val : (int64_t)sr * NSBINANG + aw);
+      if constexpr (BIN) {
+        // every counted event takes a slot of its coarse run (weight 0 when
+        // not counted)
+        const int cb = (int)(addr >> LM_CSHIFT);
+        const uint32_t pos = cbase[cb] + atomicAdd(&crank[cb], 1u);
+        if ((int64_t)pos < p.stage_cap)  // the runs were counted for exactly these events
+          p.stage[pos] = (uint32_t)(addr & ((1 << LM_CSHIFT) - 1)) |
+                         ((uint32_t)((wv >> 30) & 1) << LM_CSHIFT) | ((okN ? N : 0u) << (LM_CSHIFT + 1));
+        if (!okN) continue;
+      }
       const int k = itag - itag0;
       const bool inwin = k >= 0 && k < LM_HW;
       if ((wv >> 30) & 1) {
-        atomicAdd(p.psino + addr, N);
+        if constexpr (!BIN) atomicAdd(p.psino + addr, N);
         if (inwin) atomicAdd(&hcp[k], N); else atomicAdd(p.phc + itag, N);
         if (inwin) {
           atomicAdd(&mzr[k], (unsigned long long)sr);
@@ -314,7 +400,7 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
           atomicAdd(p.pvs + (int64_t)(itag >> VTIME) * SEG0 * NSBINS + sr * NSBINS + w,
                     N << (8 * (a == 126)));
       } else {
-        atomicAdd(p.dsino + addr, N);
+        if constexpr (!BIN) atomicAdd(p.dsino + addr, N);
         if (inwin) atomicAdd(&hcd[k], N); else atomicAdd(p.dhc + itag, N);
         short2 rr = p.rno[si], cr = p.s2cF[a + NSANGLES * w];
         atomicAdd(p.fan + NCRS * rr.x + cr.x, N);
@@ -523,6 +609,174 @@ __global__ void k_philox_kat(const uint32_t *ctr, const uint32_t *key, uint32_t 
   for (int i = 0; i < 4; i++) out[i] = c[i];
 }
 
+// ---- LDS-privatised sinogram counts (one replicate, no resampling) ----
+// Pass 1: events per bucket of LM_BBINS bins and per coarse bucket, counted
+// in LDS over the blocks blockIdx.x, + gridDim.x, ... (the events k_lm_hist
+// counts: in the frame, address in range), added to gcnt[nbk] and gccnt[ncb]
+// once per workgroup.
+__global__ __launch_bounds__(LM_THREADS) void k_lm_bin_count(LmDev p, int64_t nblk, int nbk,
+                                                             uint32_t *gcnt, uint32_t *gccnt) {
+  extern __shared__ uint32_t bcnt[];
+  __shared__ uint32_t ccnt[LM_MAXCB];
+  __shared__ int32_t lut[NSN1];
+  __shared__ int32_t wave_last[LM_WAVES];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int i = tid; i < nbk; i += LM_THREADS) bcnt[i] = 0;
+  for (int i = tid; i < p.ncb; i += LM_THREADS) ccnt[i] = 0;
+  for (int i = tid; i < NSN1; i += LM_THREADS) lut[i] = p.lut[i];
+  const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1);
+  for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const int64_t w0 = blk * LM_BLOCK + (int64_t)wave * (LM_PER_LANE * 64);
+    int32_t wd[LM_PER_LANE];
+    __syncthreads();  // LDS set up; wave_last of the previous block read
+    const int32_t wl = lm_load(p, w0, lane, wd);
+    if (lane == 0) wave_last[wave] = wl;
+    __syncthreads();
+    int32_t c = p.blk_carry[blk];
+    for (int k = 0; k < wave; k++)
+      if (wave_last[k] >= 0) c = wave_last[k];
+    for (int j = 0; j < LM_PER_LANE; j++) {
+      const int32_t word = wd[j];
+      const int32_t ms = lm_tag_at(word, c, le);
+      const int itag = (ms < 0 ? 0 : ms - p.toff) / ITIME;
+      if (word > 0 && itag >= p.tstart && itag < p.tstop &&
+          (word & 0x3fffffff) < NSN1 * NSBINANG) {
+        const int64_t bin = lm_bin(p.span, word & 0x3fffffff, lut);
+        atomicAdd(&bcnt[bin >> LM_BSHIFT], 1u);
+        atomicAdd(&ccnt[bin >> LM_CSHIFT], 1u);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < nbk; i += LM_THREADS)
+    if (bcnt[i]) atomicAdd(gcnt + i, bcnt[i]);
+  for (int i = tid; i < p.ncb; i += LM_THREADS)
+    if (ccnt[i]) atomicAdd(gccnt + i, ccnt[i]);
+}
+
+// Slot ranges from counts: bucket b's events go to [bstart[b], bstart[b + 1]);
+// bcur = bstart (the cursors the next pass takes slots from). One block.
+__global__ __launch_bounds__(1024) void k_lm_bin_scan(const uint32_t *gcnt, int nbk,
+                                                      uint32_t *bstart, uint32_t *bcur) {
+  __shared__ uint32_t t[1024];
+  __shared__ uint32_t carry;
+  const int tid = threadIdx.x;
+  if (tid == 0) carry = 0;
+  for (int b0 = 0; b0 < nbk; b0 += 1024) {
+    __syncthreads();
+    const int i = b0 + tid;
+    const uint32_t v = i < nbk ? gcnt[i] : 0u;
+    t[tid] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {  // inclusive scan
+      const uint32_t x = tid >= o ? t[tid - o] : 0u;
+      __syncthreads();
+      t[tid] += x;
+      __syncthreads();
+    }
+    const uint32_t ex = carry + t[tid] - v;
+    if (i < nbk) {
+      bstart[i] = ex;
+      bcur[i] = ex;
+    }
+    __syncthreads();
+    if (tid == 1023) carry += t[1023];
+  }
+  __syncthreads();
+  if (tid == 0) bstart[nbk] = carry;
+}
+
+// Pass 3: tiles of LM_TILE events of the coarse stage (s1, coarse buckets
+// [cstart[c], cstart[c + 1])) sorted in LDS by bucket and written to the
+// buckets' ranges of s2 (cursors fcur) in runs, as (bin within bucket) |
+// prompt << LM_BSHIFT | N << (LM_BSHIFT + 1).
+__global__ __launch_bounds__(LM_THREADS) void k_lm_bin_part(const uint32_t *__restrict__ s1,
+                                                            const uint32_t *__restrict__ cstart,
+                                                            int ncb, int nbk, uint32_t *fcur,
+                                                            uint32_t *s2) {
+  extern __shared__ uint32_t fc[];  // bucket counters of the tile's window, then run bases
+  __shared__ uint32_t pay[LM_TILE];
+  __shared__ uint16_t rk[LM_TILE], fo[LM_TILE];
+  __shared__ uint32_t cs[LM_MAXCB + 1];
+  __shared__ int win[3];  // first coarse bucket, last coarse bucket, first bucket
+  const int tid = threadIdx.x;
+  const uint32_t total = cstart[ncb];
+  const int64_t t0 = (int64_t)blockIdx.x * LM_TILE;
+  if (t0 >= (int64_t)total) return;  // block-uniform, before any barrier
+  const int nt = (int)((int64_t)total - t0 < LM_TILE ? (int64_t)total - t0 : LM_TILE);
+  for (int i = tid; i <= ncb; i += LM_THREADS) cs[i] = cstart[i];
+  __syncthreads();
+  // coarse bucket of stage position e: the largest c with cs[c] <= e
+  auto coarse = [&](uint32_t e, int lo, int hi) {
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (cs[mid] <= e) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+  };
+  if (tid == 0) {
+    win[0] = coarse((uint32_t)t0, 0, ncb - 1);
+    win[1] = coarse((uint32_t)(t0 + nt - 1), 0, ncb - 1);
+  }
+  __syncthreads();
+  const int c0 = win[0], c1 = win[1], f0 = c0 * LM_FPC;
+  const int f1 = (c1 + 1) * LM_FPC < nbk ? (c1 + 1) * LM_FPC : nbk;
+  for (int i = tid; i < f1 - f0; i += LM_THREADS) fc[i] = 0;
+  __syncthreads();
+  for (int i = tid; i < nt; i += LM_THREADS) {
+    const uint32_t v = s1[t0 + i];
+    const int cb = coarse((uint32_t)(t0 + i), c0, c1);
+    const int f = cb * LM_FPC + (int)((v & ((1u << LM_CSHIFT) - 1)) >> LM_BSHIFT) - f0;
+    rk[i] = (uint16_t)atomicAdd(&fc[f], 1u);
+    fo[i] = (uint16_t)f;
+    pay[i] = (v & (LM_BBINS - 1)) | ((v >> LM_CSHIFT) << LM_BSHIFT);
+  }
+  __syncthreads();
+  for (int i = tid; i < f1 - f0; i += LM_THREADS) {
+    const uint32_t n = fc[i];
+    fc[i] = n ? atomicAdd(fcur + f0 + i, n) : 0u;
+  }
+  __syncthreads();
+  for (int i = tid; i < nt; i += LM_THREADS) {
+    const uint32_t pos = fc[fo[i]] + rk[i];
+    if (pos < total) s2[pos] = pay[i];  // the ranges were counted for exactly these events
+  }
+}
+
+// Pass 4: one block per bucket counts its staged events in LDS and adds the
+// bucket's bins to the sinograms (each bin belongs to one block: no atomics
+// on the outputs, and buckets without events are not touched).
+__global__ __launch_bounds__(1024) void k_lm_bin_apply(const uint32_t *__restrict__ stage,
+                                                       const uint32_t *__restrict__ bstart,
+                                                       int64_t nbins, uint32_t *psino,
+                                                       uint32_t *dsino) {
+  __shared__ uint32_t pc[LM_BBINS], dc[LM_BBINS];
+  const int tid = threadIdx.x;
+  const uint32_t s0 = bstart[blockIdx.x], s1 = bstart[blockIdx.x + 1];
+  if (s0 == s1) return;  // block-uniform, before any barrier
+  for (int i = tid; i < LM_BBINS; i += 1024) {
+    pc[i] = 0;
+    dc[i] = 0;
+  }
+  __syncthreads();
+  for (uint32_t e = s0 + tid; e < s1; e += 1024) {
+    const uint32_t v = stage[e];
+    const uint32_t n = v >> (LM_BSHIFT + 1);
+    if (n) atomicAdd(((v >> LM_BSHIFT) & 1u ? pc : dc) + (v & (LM_BBINS - 1)), n);
+  }
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x << LM_BSHIFT;
+  const int lim = (int)(nbins - base < LM_BBINS ? nbins - base : LM_BBINS);
+  for (int i = tid; i < lim; i += 1024) {
+    if (pc[i]) psino[base + i] += pc[i];
+    if (dc[i]) dsino[base + i] += dc[i];
+  }
+}
+
+__global__ void k_lm_zero_u32(uint32_t *p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
 }  // namespace nipet
 
 using namespace nipet;
@@ -543,12 +797,21 @@ struct NipetLm {
   unsigned int sgl_cap = 0;
   int32_t *ssrb_off = nullptr, *ssrb_list = nullptr;  // sinograms of each SSRB plane
   int64_t base = 0;  // global index of the next word
+  // LDS-privatised sinogram counts: buckets of LM_BBINS output bins (nbk),
+  // coarse buckets of 2^LM_CSHIFT bins (ncb): counts, slot ranges, cursors
+  int64_t nbins = 0;
+  int nbk = 0, ncb = 0;
+  uint32_t *gcnt = nullptr, *bstart = nullptr, *bcur = nullptr;
+  uint32_t *gccnt = nullptr, *cstart = nullptr, *ccur = nullptr;
+  uint32_t *stage = nullptr, *stage2 = nullptr;  // coarse order, bucket order
+  int64_t stage_cap = 0;
 };
 
 static void lm_free(NipetLm *h) {
   void *ps[] = {h->lut, h->rno, h->s2cF, h->cdf, h->carry, h->blk_last, h->blk_carry,
                 h->first_nz, h->sgl_idx, h->sgl_word, h->sgl_ms, h->sgl_count, h->bad,
-                h->ssrb_off, h->ssrb_list};
+                h->ssrb_off, h->ssrb_list, h->gcnt, h->bstart, h->bcur, h->gccnt,
+                h->cstart, h->ccur, h->stage, h->stage2};
   for (void *q : ps) (void)hipFree(q);
 }
 
@@ -621,6 +884,17 @@ static int lm_build(const NipetLmConfig *c, NipetLm *h) {
   NIPET_HIP(hipMalloc(&h->sgl_count, sizeof(unsigned int)));
   NIPET_HIP(hipMalloc(&h->bad, sizeof(unsigned int)));
   NIPET_HIP(hipMalloc(&h->first_nz, (size_t)c->nitag * NBUCKTS * sizeof(unsigned long long)));
+  h->nbins = (int64_t)nsino * NSBINANG;
+  h->nbk = (int)((h->nbins + LM_BBINS - 1) >> LM_BSHIFT);
+  h->ncb = (int)((h->nbins + (1 << LM_CSHIFT) - 1) >> LM_CSHIFT);
+  if (c->nrep == 1 && c->btp != 1) {  // LDS-privatised sinogram counts
+    NIPET_HIP(hipMalloc(&h->gcnt, (size_t)h->nbk * sizeof(uint32_t)));
+    NIPET_HIP(hipMalloc(&h->bstart, (size_t)(h->nbk + 1) * sizeof(uint32_t)));
+    NIPET_HIP(hipMalloc(&h->bcur, (size_t)h->nbk * sizeof(uint32_t)));
+    NIPET_HIP(hipMalloc(&h->gccnt, (size_t)h->ncb * sizeof(uint32_t)));
+    NIPET_HIP(hipMalloc(&h->cstart, (size_t)(h->ncb + 1) * sizeof(uint32_t)));
+    NIPET_HIP(hipMalloc(&h->ccur, (size_t)h->ncb * sizeof(uint32_t)));
+  }
   NIPET_HIP(hipMemcpy(h->lut, lut.data(), NSN1 * sizeof(int32_t), hipMemcpyHostToDevice));
   NIPET_HIP(hipMemcpy(h->rno, c->sn1_rno, NSN1 * sizeof(short2), hipMemcpyHostToDevice));
   NIPET_HIP(hipMemcpy(h->s2cF, c->s2cF, NSBINANG * sizeof(short2), hipMemcpyHostToDevice));
@@ -733,7 +1007,16 @@ void nipet_lm_destroy(NipetLm *h) {
   delete h;
 }
 
-static int grow(NipetLm *h, int64_t nblk, unsigned int sgl) {
+static int grow(NipetLm *h, int64_t nblk, unsigned int sgl, int64_t stage) {
+  if (stage > h->stage_cap) {
+    (void)hipFree(h->stage);
+    (void)hipFree(h->stage2);
+    h->stage = h->stage2 = nullptr;
+    h->stage_cap = 0;
+    NIPET_HIP(hipMalloc(&h->stage, (size_t)stage * sizeof(uint32_t)));
+    NIPET_HIP(hipMalloc(&h->stage2, (size_t)stage * sizeof(uint32_t)));
+    h->stage_cap = stage;
+  }
   if (nblk > h->blk_cap) {
     (void)hipFree(h->blk_last);
     (void)hipFree(h->blk_carry);
@@ -783,7 +1066,10 @@ int nipet_lm_chunk(NipetLm *h, const int32_t *words, int64_t n, const NipetLmOut
   // singles list: at most one entry per word; size it generously for the
   // chunk, entries beyond the capacity are reported as an error
   unsigned int cap = (unsigned int)std::min<int64_t>(std::max<int64_t>(n / 32, 1 << 20), 1 << 28);
-  int rc = grow(h, nblk, cap);
+  // one replicate without resampling: sinogram counts privatised in LDS per
+  // bucket (staged events need a slot per word of the chunk)
+  const bool bin = h->gcnt && n <= (int64_t)UINT32_MAX;
+  int rc = grow(h, nblk, cap, bin ? n : 0);
   if (rc) return rc;
   prof_begin(PK_HIST, st);
   k_lm_tags<<<(unsigned)nblk, LM_THREADS, 0, st>>>(words, n, h->blk_last, h->sgl_count);
@@ -823,11 +1109,28 @@ int nipet_lm_chunk(NipetLm *h, const int32_t *words, int64_t n, const NipetLmOut
   p.sgl_count = h->sgl_count;
   p.sgl_cap = h->sgl_cap;
   p.bad = h->bad;
+  p.ccur = h->ccur;
+  p.stage = h->stage;
+  p.stage_cap = bin ? n : 0;
+  p.ncb = h->ncb;
   if (c.btp == 1) {
     if (c.nrep == 1)
       k_lm_hist<false, true><<<(unsigned)nblk, LM_THREADS, 0, st>>>(p);
     else
       k_lm_hist<true, true><<<(unsigned)nblk, LM_THREADS, 0, st>>>(p);
+  } else if (bin) {
+    k_lm_zero_u32<<<(h->nbk + 255) / 256, 256, 0, st>>>(h->gcnt, h->nbk);
+    k_lm_zero_u32<<<(h->ncb + 255) / 256, 256, 0, st>>>(h->gccnt, h->ncb);
+    k_lm_bin_count<<<(unsigned)std::min<int64_t>(nblk, 2048), LM_THREADS,
+                     (size_t)h->nbk * sizeof(uint32_t), st>>>(p, nblk, h->nbk, h->gcnt, h->gccnt);
+    k_lm_bin_scan<<<1, 1024, 0, st>>>(h->gcnt, h->nbk, h->bstart, h->bcur);
+    k_lm_bin_scan<<<1, 1024, 0, st>>>(h->gccnt, h->ncb, h->cstart, h->ccur);
+    k_lm_hist<false, false, true><<<(unsigned)nblk, LM_THREADS, 0, st>>>(p);
+    k_lm_bin_part<<<(unsigned)((n + LM_TILE - 1) / LM_TILE), LM_THREADS,
+                    (size_t)h->nbk * sizeof(uint32_t), st>>>(h->stage, h->cstart, h->ncb, h->nbk,
+                                                             h->bcur, h->stage2);
+    k_lm_bin_apply<<<(unsigned)h->nbk, 1024, 0, st>>>(h->stage2, h->bstart, h->nbins, o->psino,
+                                                       o->dsino);
   } else if (c.nrep == 1) {
     k_lm_hist<false, false><<<(unsigned)nblk, LM_THREADS, 0, st>>>(p);
   } else {

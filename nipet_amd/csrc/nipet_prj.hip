@@ -697,15 +697,19 @@ __global__ __launch_bounds__(fpg_threads<G>()) void k_fprj_g(GeomView g, const f
 // through a private LDS buffer with one coalesced global float atomic per
 // (segment, plane, replicate).
 
+// Segments per run (SPW): compiled for two run lengths per width; the
+// autotuner picks one (nipet_tune_set_run). Defaults, measured on MI355X:
+// 8 for one image, 4 for two, 6 for four.
 template <int NB>
-constexpr int bp_spw() { return NB == 1 ? 8 : (NB == 2 ? 4 : 6); }  // measured per width
+constexpr int bp_spw_a() { return NB == 1 ? 8 : (NB == 2 ? 4 : 6); }
+template <int NB>
+constexpr int bp_spw_b() { return NB == 1 ? 6 : (NB == 2 ? 6 : 4); }
 
-template <int NB>
+template <int NB, int SPW>
 __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g, float *__restrict__ img,
                                                      const float *__restrict__ sino,
                                                      const int32_t *__restrict__ subs, Batch bt,
                                                      int by_bin) {
-  constexpr int SPW = bp_spw<NB>();
   static_assert(SPW <= 8, "the run word holds the boundary bits and steps of 8 segments");
   constexpr int NWAVE = BP_THREADS / 64;
   __shared__ __align__(16) float srow[(MAXNSN / NB + 1) * NB];   // sinogram row + zero row
@@ -1066,6 +1070,19 @@ using namespace nipet;
 // Largest replicate sub-batch per kernel instance (4, 2 or 1), set by the
 // autotuner through nipet_tune_set; process-wide.
 static std::atomic<int> g_width_fp{4}, g_width_bp{4};
+// BP segments per run for widths 1, 2, 4 (one of bp_spw_a / bp_spw_b)
+static std::atomic<int> g_run_bp[3] = {{bp_spw_a<1>()}, {bp_spw_a<2>()}, {bp_spw_a<4>()}};
+
+// One BP launch of width NB with the tuned run length.
+template <int NB>
+static void launch_bprj_w(const Geom *g, float *img, const float *sino, const int32_t *subs,
+                          int nprj, Batch bt, int by_bin, hipStream_t st) {
+  const int run = g_run_bp[NB == 1 ? 0 : (NB == 2 ? 1 : 2)].load();
+  if (run == bp_spw_b<NB>())
+    k_bprj<NB, bp_spw_b<NB>()><<<nprj, BP_THREADS, 0, st>>>(view(g), img, sino, subs, bt, by_bin);
+  else
+    k_bprj<NB, bp_spw_a<NB>()><<<nprj, BP_THREADS, 0, st>>>(view(g), img, sino, subs, bt, by_bin);
+}
 
 template <int MODE>
 static void launch_fprj(const Geom *g, float *sino, const float *img, const int32_t *subs,
@@ -1098,13 +1115,13 @@ static void launch_bprj(const Geom *g, float *img, const float *sino, const int3
     const int rem = nb - b0;
     const Batch bt{nb, b0};
     if (rem >= 4 && wmax >= 4) {
-      k_bprj<4><<<nprj, BP_THREADS, 0, st>>>(view(g), img, sino, subs, bt, by_bin);
+      launch_bprj_w<4>(g, img, sino, subs, nprj, bt, by_bin, st);
       b0 += 4;
     } else if (rem >= 2 && wmax >= 2) {
-      k_bprj<2><<<nprj, BP_THREADS, 0, st>>>(view(g), img, sino, subs, bt, by_bin);
+      launch_bprj_w<2>(g, img, sino, subs, nprj, bt, by_bin, st);
       b0 += 2;
     } else {
-      k_bprj<1><<<nprj, BP_THREADS, 0, st>>>(view(g), img, sino, subs, bt, by_bin);
+      launch_bprj_w<1>(g, img, sino, subs, nprj, bt, by_bin, st);
       b0 += 1;
     }
   }
@@ -1324,6 +1341,34 @@ int nipet_tune_set(int kernel, int width) {
   return NIPET_OK;
 }
 
+int nipet_tune_set_run(int width, int spw) {
+  const int k = width == 1 ? 0 : (width == 2 ? 1 : (width == 4 ? 2 : -1));
+  const int a = k == 0 ? bp_spw_a<1>() : (k == 1 ? bp_spw_a<2>() : bp_spw_a<4>());
+  const int b = k == 0 ? bp_spw_b<1>() : (k == 1 ? bp_spw_b<2>() : bp_spw_b<4>());
+  if (k < 0 || (spw != a && spw != b)) {
+    set_error("nipet_tune_set_run: width must be 1, 2 or 4 and the run length one of the "
+              "compiled ones (%d or %d)", a, b);
+    return NIPET_EINVAL;
+  }
+  g_run_bp[k].store(spw);
+  return NIPET_OK;
+}
+
+int nipet_tune_get_run(int width, int *spw, int *alt) {
+  const int k = width == 1 ? 0 : (width == 2 ? 1 : (width == 4 ? 2 : -1));
+  if (k < 0 || !spw) {
+    set_error("nipet_tune_get_run: width must be 1, 2 or 4");
+    return NIPET_EINVAL;
+  }
+  *spw = g_run_bp[k].load();
+  if (alt) {
+    const int a = k == 0 ? bp_spw_a<1>() : (k == 1 ? bp_spw_a<2>() : bp_spw_a<4>());
+    const int b = k == 0 ? bp_spw_b<1>() : (k == 1 ? bp_spw_b<2>() : bp_spw_b<4>());
+    *alt = *spw == a ? b : a;
+  }
+  return NIPET_OK;
+}
+
 int nipet_tune_get(int kernel, int *width) {
   if ((kernel != NIPET_PROF_FPRJ && kernel != NIPET_PROF_BPRJ) || !width) {
     set_error("nipet_tune_get: kernel must be FPRJ or BPRJ");
@@ -1475,7 +1520,7 @@ int nipet_osem_subset_sens(const NipetGeom *g, float *img, const uint16_t *promp
   // both images through one traversal of the rays (independent of the
   // batch autotuner's width)
   prof_begin(PK_BPRJ_PAIR, st);
-  k_bprj<2><<<nprj, BP_THREADS, 0, st>>>(view(g), work_img2, work_sino2, subs, Batch{2, 0}, 0);
+  launch_bprj_w<2>(g, work_img2, work_sino2, subs, nprj, Batch{2, 0}, 0, st);
   prof_end(PK_BPRJ_PAIR, st);
   prof_begin(PK_OSEM_UPDATE, st);
   k_osem_update_pair<<<ew_blocks(NVOX), 256, 0, st>>>(img, sens, (const float2 *)work_img2, mask,

@@ -19,8 +19,8 @@ normalisation (hence the 14 sensitivity images) run as one batch:
   prompts [group][bin][837][4]): a kernel launch covers one group, so every
   cache line it touches holds only that group's replicates;
 * the replicate sub-batch width of the FP and BP kernels (4, 2 or 1 images per
-  launch) is picked by timing both kernels on the problem's own geometry and
-  group size (``autotune``).
+  launch), and the BP's run length at each width, are picked by timing the
+  kernels on the problem's own geometry and group size (``autotune``).
 """
 import numpy as np
 
@@ -50,10 +50,13 @@ _TUNED = {}
 
 def autotune(state, widths=(4, 2, 1), repeats=2, probe=4):
     """Time the emission FP and the BP kernels on the first subset of
-    ``state`` (an ``OsemBatch``) for each sub-batch width, over a probe batch
-    of min(nb, ``probe``) replicates, and keep the fastest width per kernel.
-    Results are cached per (geometry, subset size) for the process. Returns
-    {'fprj': w, 'bprj': w, 'ms': {(kernel, w): ms per launch}}."""
+    ``state`` (an ``OsemBatch``) over a probe batch of min(nb, ``probe``)
+    replicates, and keep the fastest shape per kernel: the FP sub-batch width,
+    and the BP sub-batch width together with its run length (segments per
+    wave run: each width has two compiled run lengths). Results are cached
+    per (geometry, subset size) for the process. Returns {'fprj': w, 'bprj': w,
+    'bprj_run': {w: segments}, 'ms': {(kernel, w[, run]): ms per image}}."""
+    import ctypes as ct
     torch = _torch()
     L = _lib.lib()
     key = (id(state.geom), len(state.subsets[0]))
@@ -61,6 +64,8 @@ def autotune(state, widths=(4, 2, 1), repeats=2, probe=4):
         res = _TUNED[key]
         _lib.check(L.nipet_tune_set(_lib.PROF_FPRJ, res['fprj']), 'nipet_tune_set')
         _lib.check(L.nipet_tune_set(_lib.PROF_BPRJ, res['bprj']), 'nipet_tune_set')
+        for w, run in res['bprj_run'].items():
+            _lib.check(L.nipet_tune_set_run(w, run), 'nipet_tune_set_run')
         return res
     st = _lib.stream_handle()
     subs = state.subsets[0]
@@ -69,32 +74,55 @@ def autotune(state, widths=(4, 2, 1), repeats=2, probe=4):
     img = torch.ones((320, 320, 127, nb), dtype=torch.float32, device='cuda')
     sino = torch.empty((nprj, state.geom.nsn, nb), dtype=torch.float32, device='cuda')
     bimg = torch.empty_like(img)
-    res = {'ms': {}}
-    for kern, name in ((_lib.PROF_FPRJ, 'fprj'), (_lib.PROF_BPRJ, 'bprj')):
-        def launch():
-            if kern == _lib.PROF_FPRJ:
-                rc = L.nipet_fprj_b(state.geom.handle, _lib.ptr(sino), _lib.ptr(img),
-                                    _lib.ptr(subs), nprj, nb, 0, st)
-            else:
-                rc = L.nipet_bprj_b(state.geom.handle, _lib.ptr(bimg), _lib.ptr(sino),
-                                    _lib.ptr(subs), nprj, nb, st)
-            _lib.check(rc, name)
-        best = None
-        for w in widths:
-            _lib.check(L.nipet_tune_set(kern, int(w)), 'nipet_tune_set')
+    res = {'ms': {}, 'bprj_run': {}}
+
+    def timed(launch, n):
+        launch()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(repeats):
             launch()
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
-            for _ in range(repeats):
-                launch()
-            e1.record()
-            e1.synchronize()
-            ms = e0.elapsed_time(e1) / repeats
-            res['ms'][(name, w)] = ms
-            if best is None or ms < best[1]:
-                best = (w, ms)
-        _lib.check(L.nipet_tune_set(kern, best[0]), 'nipet_tune_set')
-        res[name] = best[0]
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) / repeats / n
+
+    def fprj(n):
+        return lambda: _lib.check(L.nipet_fprj_b(state.geom.handle, _lib.ptr(sino), _lib.ptr(img),
+                                                 _lib.ptr(subs), nprj, n, 0, st), 'nipet_fprj_b')
+
+    def bprj(n):
+        return lambda: _lib.check(L.nipet_bprj_b(state.geom.handle, _lib.ptr(bimg),
+                                                 _lib.ptr(sino), _lib.ptr(subs), nprj, n, st),
+                                  'nipet_bprj_b')
+    best = None
+    for w in widths:
+        n = min(w, nb)
+        _lib.check(L.nipet_tune_set(_lib.PROF_FPRJ, int(w)), 'nipet_tune_set')
+        ms = timed(fprj(n), n)
+        res['ms'][('fprj', w)] = ms
+        if best is None or ms < best[1]:
+            best = (w, ms)
+    _lib.check(L.nipet_tune_set(_lib.PROF_FPRJ, best[0]), 'nipet_tune_set')
+    res['fprj'] = best[0]
+    best = None
+    for w in widths:
+        n = min(w, nb)
+        _lib.check(L.nipet_tune_set(_lib.PROF_BPRJ, int(w)), 'nipet_tune_set')
+        cur, alt = ct.c_int(), ct.c_int()
+        _lib.check(L.nipet_tune_get_run(int(w), ct.byref(cur), ct.byref(alt)), 'tune_get_run')
+        wbest = None
+        for run in (cur.value, alt.value):
+            _lib.check(L.nipet_tune_set_run(int(w), run), 'nipet_tune_set_run')
+            ms = timed(bprj(n), n)
+            res['ms'][('bprj', w, run)] = ms
+            if wbest is None or ms < wbest[1]:
+                wbest = (run, ms)
+        _lib.check(L.nipet_tune_set_run(int(w), wbest[0]), 'nipet_tune_set_run')
+        res['bprj_run'][int(w)] = wbest[0]
+        if best is None or wbest[1] < best[1]:
+            best = (w, wbest[1])
+    _lib.check(L.nipet_tune_set(_lib.PROF_BPRJ, best[0]), 'nipet_tune_set')
+    res['bprj'] = best[0]
     _TUNED[key] = res
     return res
 

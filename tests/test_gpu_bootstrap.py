@@ -92,10 +92,23 @@ def test_widths_do_not_change_images(torch, params, data):
         for k in (_lib.PROF_FPRJ, _lib.PROF_BPRJ):
             _lib.check(_lib.lib().nipet_tune_set(k, w), 'tune')
         out.append(osem_run_batch(st, 1))
+    # the other compiled BP run length of each width
+    import ctypes as ct
+    runs = {}
+    for w in (4, 2, 1):
+        cur, alt = ct.c_int(), ct.c_int()
+        _lib.check(_lib.lib().nipet_tune_get_run(w, ct.byref(cur), ct.byref(alt)), 'get_run')
+        runs[w] = (cur.value, alt.value)
+        _lib.check(_lib.lib().nipet_tune_set_run(w, alt.value), 'set_run')
     for k in (_lib.PROF_FPRJ, _lib.PROF_BPRJ):
         _lib.check(_lib.lib().nipet_tune_set(k, 4), 'tune')
+    out.append(osem_run_batch(st, 1))
+    for w, (cur, alt) in runs.items():
+        _lib.check(_lib.lib().nipet_tune_set_run(w, cur), 'set_run')
     assert _rel(torch, out[0], out[1]) <= 1e-6
+    assert _rel(torch, out[0], out[2]) <= 1e-6
     assert _lib.lib().nipet_tune_set(_lib.PROF_FPRJ, 3) != 0
+    assert _lib.lib().nipet_tune_set_run(4, 5) != 0
 
 
 def test_frames_batch_equals_sequential(torch, params, data):

@@ -88,6 +88,22 @@ def test_hist_bit_exact(torch, params, stream, btp):
     assert g['phc'].sum() == g['psino'].sum() and g['dhc'].sum() == g['dsino'].sum()
 
 
+@pytest.mark.parametrize('span', [1, 0])
+def test_hist_span1_and_ssrb(torch, params, stream, span):
+    """Span-1 (4084 sinograms) and SSRB (127) sessions: the bucketed LDS counts
+    cover 21,609 and 672 buckets of 16,384 bins; plain and bootstrap, bit-exact
+    against the oracle."""
+    from oracle import oracle as O
+    toff, nitag = _toff_nitag(stream)
+    w = stream[:900_000]
+    for btp in (0, 2):
+        h, g = _gpu(torch, params, w, toff, nitag, span=span, btp=btp, seed=31)
+        o = O.lm_hist(w, params, toff, nitag, span=span, btp=btp, seed=31)
+        for k in ('psino', 'dsino', 'phc', 'dhc'):
+            assert np.array_equal(g[k].astype(np.uint32), o[k].T), (span, btp, k)
+        assert g['psino'].shape[0] == (4084 if span == 1 else 127) * 86688
+
+
 def test_hist_chunked_frame_window(torch, params, stream):
     from oracle import oracle as O
     toff, nitag = _toff_nitag(stream)
