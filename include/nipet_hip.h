@@ -243,7 +243,8 @@ typedef struct {
                             but the last must then hold a multiple of 8192 words);
                             2: parametric (Poisson weights), lm/src/hst.cu:110-139 */
   double btprt;          /* Poisson mean of the bootstrap weights (Cnt['BTPRT']) */
-  uint64_t seed;         /* Philox4x32-10 key */
+  uint64_t seed;         /* Philox4x32-10 key; btp 2 draws word g of replicate r from
+                            output word g % 4 of counter (g / 4, rep0 + r, 0) */
   int nrep;              /* replicates 1..64; nrep > 1 stores [bin][nrep] */
   int rep0;              /* replicate index of the first stored replicate (Philox counter
                             word 2 = rep0 + r), so sessions can split a replicate set */

@@ -60,6 +60,8 @@ constexpr int LM_CSHIFT = 20;
 constexpr int LM_FPC = 1 << (LM_CSHIFT - LM_BSHIFT);  // buckets per coarse bucket
 constexpr int LM_MAXCB = (NSN1_BINS_MAX + (1 << LM_CSHIFT) - 1) >> LM_CSHIFT;
 constexpr int LM_TILE = 8192;
+static_assert(((NSN1_BINS_MAX + LM_BBINS - 1) >> LM_BSHIFT) < 65536 && LM_TILE <= 65536,
+              "k_lm_bin_part packs a bucket index and a rank into 16 bits each");
 constexpr int32_t TMSK = 0x1fffffff;
 
 __host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
@@ -225,6 +227,23 @@ __device__ inline uint32_t poisson_n(uint32_t u, const uint32_t (&c8)[LM_FASTK],
   return N;
 }
 
+// Uniform draw of the parametric bootstrap for word g and replicate r: output
+// word (g mod 4) of Philox4x32-10(counter (g / 4 lo, hi, r, 0), key (k0, k1)).
+// One generator call serves four consecutive words of a replicate.
+__device__ inline uint32_t boot_pick(const uint32_t (&c)[4], uint64_t g) {
+  const int m = (int)(g & 3);
+  return m == 0 ? c[0] : (m == 1 ? c[1] : (m == 2 ? c[2] : c[3]));
+}
+__device__ inline void boot_quad(uint32_t (&c)[4], uint64_t g, uint32_t r, uint32_t k0,
+                                 uint32_t k1) {
+  const uint64_t q = g >> 2;
+  c[0] = (uint32_t)q;
+  c[1] = (uint32_t)(q >> 32);
+  c[2] = r;
+  c[3] = 0u;
+  philox4x32_10(c, k0, k1);
+}
+
 // REP = false: one event per lane (nrep == 1).
 // REP = true: the wave walks its events one by one, lane r = replicate r.
 // NP: non-parametric bootstrap (btp = 1): the block's event words are
@@ -232,8 +251,9 @@ __device__ inline uint32_t poisson_n(uint32_t u, const uint32_t (&c8)[LM_FASTK],
 // event is replaced by a draw from them.
 // BIN (one replicate, no resampling): the prompt / delayed sinogram counts are
 // not added here. The block counts its events per coarse bucket, reserves one
-// run of slots per coarse bucket, and stages every counted event in its run
-// (the coarse slot ranges were counted by k_lm_bin_count); k_lm_bin_part and
+// run of slots per coarse bucket, sorts its counted events into those runs in
+// LDS and writes the runs out with consecutive lanes on consecutive slots (the
+// coarse slot ranges were counted by k_lm_bin_count); k_lm_bin_part and
 // k_lm_bin_apply then count them in LDS.
 template <bool REP, bool NP, bool BIN = false>
 __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
@@ -247,6 +267,7 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
   __shared__ unsigned long long mzr[LM_HW], mzm[LM_HW];
   __shared__ int32_t wave_last[LM_WAVES];
   __shared__ uint32_t crank[BIN ? LM_MAXCB : 1], cbase[BIN ? LM_MAXCB : 1];
+  __shared__ uint32_t stg[BIN ? LM_BLOCK : 1];  // the block's staged events, by coarse bucket
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nr = REP ? p.nrep : 1;
   for (int i = tid; i < NSN1; i += LM_THREADS) lut[i] = p.lut[i];
@@ -305,7 +326,9 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
   const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1);
   if constexpr (BIN) {
     // the block's counted events per coarse bucket (the same walk and test as
-    // below), then one run of slots per coarse bucket
+    // below), then one run of slots per coarse bucket: crank becomes the
+    // run's first slot in stg (exclusive scan by wave 0), cbase the run's
+    // global slot minus that local start
     int32_t c1 = c;
     for (int j = 0; j < LM_PER_LANE; j++) {
       const int32_t word = wd[j];
@@ -316,13 +339,30 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
         atomicAdd(&crank[lm_bin(p.span, word & 0x3fffffff, lut) >> LM_CSHIFT], 1u);
     }
     __syncthreads();
-    for (int i = tid; i < p.ncb; i += LM_THREADS) {
-      const uint32_t nc = crank[i];
-      cbase[i] = nc ? atomicAdd(p.ccur + i, nc) : 0u;
-      crank[i] = 0;
+    if (wave == 0) {
+      uint32_t run = 0;
+      for (int i0 = 0; i0 < p.ncb; i0 += 64) {
+        const int i = i0 + lane;
+        const uint32_t nc = i < p.ncb ? crank[i] : 0u;
+        uint32_t x = nc;  // inclusive scan over the wave
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const uint32_t y = __shfl_up(x, o);
+          if (lane >= o) x += y;
+        }
+        const uint32_t ex = run + x - nc;
+        if (i < p.ncb) {
+          cbase[i] = (nc ? atomicAdd(p.ccur + i, nc) : 0u) - ex;
+          crank[i] = ex;
+        }
+        run += __shfl(x, 63);
+      }
     }
     __syncthreads();
   }
+  // REP: the last generator call of the wave (its word quad, the 4 draws)
+  uint64_t bq = ~0ull;
+  uint32_t bc[4] = {0u, 0u, 0u, 0u};
   for (int j = 0; j < LM_PER_LANE; j++) {
     const int32_t word = wd[j];
     const int64_t gi = w0 + j * 64 + lane;
@@ -358,10 +398,10 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
           continue;
         }
       } else if (p.btp == 2) {
-        uint64_t g = (uint64_t)(p.base + gi);
-        uint32_t cc[4] = {(uint32_t)g, (uint32_t)(g >> 32), (uint32_t)p.rep0, 0u};
-        philox4x32_10(cc, p.k0, p.k1);
-        N = poisson_n(cc[0], c8, cdf);
+        const uint64_t g = (uint64_t)(p.base + gi);
+        uint32_t cc[4];
+        boot_quad(cc, g, (uint32_t)p.rep0, p.k0, p.k1);
+        N = poisson_n(boot_pick(cc, g), c8, cdf);
       }
       const bool okN = N > 0 && N < 32;
       if constexpr (!BIN) {
@@ -375,13 +415,12 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
       const int64_t addr = p.span == 11 ? (int64_t)s11 * NSBINANG + aw
                                         : (p.span == 1 ? (int64_t)val : (int64_t)sr * NSBINANG + aw);
       if constexpr (BIN) {
-        // every counted event takes a slot of its coarse run (weight 0 when
-        // not counted)
-        const int cb = (int)(addr >> LM_CSHIFT);
-        const uint32_t pos = cbase[cb] + atomicAdd(&crank[cb], 1u);
-        if ((int64_t)pos < p.stage_cap)  // the runs were counted for exactly these events
-          p.stage[pos] = (uint32_t)(addr & ((1 << LM_CSHIFT) - 1)) |
-                         ((uint32_t)((wv >> 30) & 1) << LM_CSHIFT) | ((okN ? N : 0u) << (LM_CSHIFT + 1));
+        // every counted event takes a slot of its coarse run in LDS (weight 0
+        // when not counted); the runs go out after the walk
+        const uint32_t s = atomicAdd(&crank[(int)(addr >> LM_CSHIFT)], 1u);
+        if (s < LM_BLOCK)  // the runs were counted for exactly these events
+          stg[s] = (uint32_t)(addr & ((1 << LM_CSHIFT) - 1)) |
+                   ((uint32_t)((wv >> 30) & 1) << LM_CSHIFT) | ((okN ? N : 0u) << (LM_CSHIFT + 1));
         if (!okN) continue;
       }
       const int k = itag - itag0;
@@ -494,14 +533,14 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
         const int32_t x1 = __builtin_amdgcn_readlane(e_x1, l);
         const int32_t x2 = __builtin_amdgcn_readlane(e_x2, l);
         const uint64_t g = (uint64_t)(p.base + w0 + j * 64 + l);
-        uint32_t N = 0;
-        if (lane < nr) {
-          N = 1;
-          if (p.btp == 2) {
-            uint32_t cc[4] = {(uint32_t)g, (uint32_t)(g >> 32), (uint32_t)(p.rep0 + lane), 0u};
-            philox4x32_10(cc, p.k0, p.k1);
-            N = poisson_n(cc[0], c8, cdf);
+        uint32_t N = lane < nr ? 1u : 0u;
+        if (p.btp == 2) {
+          if ((g >> 2) != bq) {  // wave-uniform: one generator call per quad of words
+            bq = g >> 2;
+            boot_quad(bc, g, (uint32_t)(p.rep0 + lane), p.k0, p.k1);
           }
+          const uint32_t n0 = poisson_n(boot_pick(bc, g), c8, cdf);
+          N = lane < nr ? n0 : 0u;
         }
         const bool ok = N > 0 && N < 32;
         const int k = it - itag0;
@@ -532,6 +571,21 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
     }
   }
   __syncthreads();
+  if constexpr (BIN) {
+    // crank now holds the end of every run; slot s belongs to the first run
+    // that ends after it. Consecutive lanes write consecutive words of a run.
+    const int ncb = p.ncb;
+    const uint32_t nst = crank[ncb - 1];
+    for (uint32_t s = tid; s < nst; s += LM_THREADS) {
+      int lo = 0, hi = ncb - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (crank[mid] > s) hi = mid; else lo = mid + 1;
+      }
+      const uint32_t pos = cbase[lo] + s;
+      if ((int64_t)pos < p.stage_cap) p.stage[pos] = stg[s];
+    }
+  }
   // flush the LDS windows
   for (int i = tid; i < LM_HW * nr; i += LM_THREADS) {
     const int it = itag0 + i / nr;
@@ -687,23 +741,30 @@ __global__ __launch_bounds__(1024) void k_lm_bin_scan(const uint32_t *gcnt, int 
 }
 
 // Pass 3: tiles of LM_TILE events of the coarse stage (s1, coarse buckets
-// [cstart[c], cstart[c + 1])) sorted in LDS by bucket and written to the
+// [cstart[c], cstart[c + 1])) ranked in LDS by bucket and written to the
 // buckets' ranges of s2 (cursors fcur) in runs, as (bin within bucket) |
-// prompt << LM_BSHIFT | N << (LM_BSHIFT + 1).
+// prompt << LM_BSHIFT | N << (LM_BSHIFT + 1). A thread keeps its 32 events
+// and their ranks in registers; LDS holds only the tile's bucket counters.
 __global__ __launch_bounds__(LM_THREADS) void k_lm_bin_part(const uint32_t *__restrict__ s1,
                                                             const uint32_t *__restrict__ cstart,
                                                             int ncb, int nbk, uint32_t *fcur,
                                                             uint32_t *s2) {
   extern __shared__ uint32_t fc[];  // bucket counters of the tile's window, then run bases
-  __shared__ uint32_t pay[LM_TILE];
-  __shared__ uint16_t rk[LM_TILE], fo[LM_TILE];
   __shared__ uint32_t cs[LM_MAXCB + 1];
-  __shared__ int win[3];  // first coarse bucket, last coarse bucket, first bucket
+  __shared__ int win[2];  // first and last coarse bucket of the tile
+  constexpr int PER = LM_TILE / LM_THREADS;
   const int tid = threadIdx.x;
   const uint32_t total = cstart[ncb];
   const int64_t t0 = (int64_t)blockIdx.x * LM_TILE;
   if (t0 >= (int64_t)total) return;  // block-uniform, before any barrier
   const int nt = (int)((int64_t)total - t0 < LM_TILE ? (int64_t)total - t0 : LM_TILE);
+  // the whole tile is loaded first (one memory latency per tile)
+  uint32_t v[PER];
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int i = k * LM_THREADS + tid;
+    v[k] = i < nt ? s1[t0 + i] : 0u;
+  }
   for (int i = tid; i <= ncb; i += LM_THREADS) cs[i] = cstart[i];
   __syncthreads();
   // coarse bucket of stage position e: the largest c with cs[c] <= e
@@ -723,13 +784,17 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_bin_part(const uint32_t *__re
   const int f1 = (c1 + 1) * LM_FPC < nbk ? (c1 + 1) * LM_FPC : nbk;
   for (int i = tid; i < f1 - f0; i += LM_THREADS) fc[i] = 0;
   __syncthreads();
-  for (int i = tid; i < nt; i += LM_THREADS) {
-    const uint32_t v = s1[t0 + i];
-    const int cb = coarse((uint32_t)(t0 + i), c0, c1);
-    const int f = cb * LM_FPC + (int)((v & ((1u << LM_CSHIFT) - 1)) >> LM_BSHIFT) - f0;
-    rk[i] = (uint16_t)atomicAdd(&fc[f], 1u);
-    fo[i] = (uint16_t)f;
-    pay[i] = (v & (LM_BBINS - 1)) | ((v >> LM_CSHIFT) << LM_BSHIFT);
+  // rank within the bucket (< LM_TILE) | bucket within the window << 16
+  uint32_t rf[PER];
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int i = k * LM_THREADS + tid;
+    rf[k] = 0;
+    if (i < nt) {
+      const int cb = coarse((uint32_t)(t0 + i), c0, c1);
+      const int f = cb * LM_FPC + (int)((v[k] & ((1u << LM_CSHIFT) - 1)) >> LM_BSHIFT) - f0;
+      rf[k] = atomicAdd(&fc[f], 1u) | ((uint32_t)f << 16);
+    }
   }
   __syncthreads();
   for (int i = tid; i < f1 - f0; i += LM_THREADS) {
@@ -737,9 +802,14 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_bin_part(const uint32_t *__re
     fc[i] = n ? atomicAdd(fcur + f0 + i, n) : 0u;
   }
   __syncthreads();
-  for (int i = tid; i < nt; i += LM_THREADS) {
-    const uint32_t pos = fc[fo[i]] + rk[i];
-    if (pos < total) s2[pos] = pay[i];  // the ranges were counted for exactly these events
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int i = k * LM_THREADS + tid;
+    if (i < nt) {
+      const uint32_t pos = fc[rf[k] >> 16] + (rf[k] & 0xffffu);
+      if (pos < total)  // the ranges were counted for exactly these events
+        s2[pos] = (v[k] & (LM_BBINS - 1)) | ((v[k] >> LM_CSHIFT) << LM_BSHIFT);
+    }
   }
 }
 
@@ -759,10 +829,20 @@ __global__ __launch_bounds__(1024) void k_lm_bin_apply(const uint32_t *__restric
     dc[i] = 0;
   }
   __syncthreads();
-  for (uint32_t e = s0 + tid; e < s1; e += 1024) {
-    const uint32_t v = stage[e];
-    const uint32_t n = v >> (LM_BSHIFT + 1);
-    if (n) atomicAdd(((v >> LM_BSHIFT) & 1u ? pc : dc) + (v & (LM_BBINS - 1)), n);
+  // eight loads in flight per thread before their counts
+  constexpr int PER = 8;
+  for (uint32_t e0 = s0; e0 < s1; e0 += PER * 1024) {
+    uint32_t v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const uint32_t e = e0 + k * 1024 + tid;
+      v[k] = e < s1 ? stage[e] : 0u;  // weight 0: not counted
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const uint32_t n = v[k] >> (LM_BSHIFT + 1);
+      if (n) atomicAdd(((v[k] >> LM_BSHIFT) & 1u ? pc : dc) + (v[k] & (LM_BBINS - 1)), n);
+    }
   }
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x << LM_BSHIFT;

@@ -7,8 +7,8 @@ lm/src/hst.cu:110-139). Here nb reconstructions that share attenuation and
 normalisation (hence the 14 sensitivity images) run as one batch:
 
 * one list-mode pass histograms all bootstrap replicates (replicate-innermost
-  counts, Philox counter = (word index, replicate), so replicate r is the same
-  whichever session produces it), and ``nipet_remgaps_u16b`` turns them into
+  counts, Philox counter = (word index / 4, replicate), so replicate r is the
+  same whichever session produces it), and ``nipet_remgaps_u16b`` turns them into
   the batched uint16 prompts (Naw, 837, nb);
 * dynamic frames take one histogram session per time window and carry their
   own randoms + scatter (rho per frame);

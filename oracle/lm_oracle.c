@@ -19,9 +19,10 @@
  *    (lm/src/hst.cu:141-202); singles buckets (lm/src/hst.cu:215-235);
  *  - parametric bootstrap (btp = 2, lm/src/hst.cu:127-139): N ~ Poisson(btprt),
  *    counted when 0 < N < 32. Here N is drawn by inversion of a uint32 CDF
- *    table against the first output word of Philox4x32-10 (Salmon, Moraes,
- *    Dror, Shaw, SC'11) with counter (word index lo, hi, replicate, 0) and key
- *    (seed lo, seed hi), replacing the reference's clock-seeded cuRAND;
+ *    table against output word (g mod 4) of Philox4x32-10 (Salmon, Moraes,
+ *    Dror, Shaw, SC'11) with counter (g / 4 lo, hi, replicate, 0) and key
+ *    (seed lo, seed hi), g the global word index (one generator call serves
+ *    four consecutive words), replacing the reference's clock-seeded cuRAND;
  *  - non-parametric bootstrap (btp = 1, lm/src/hst.cu:110-126): an event
  *    word is replaced by an event word drawn uniformly from its patch, the
  *    8192-word block (global word index / 8192) that holds it: with u the
@@ -142,10 +143,10 @@ int64_t orc_lm_hist(const int32_t *words, int64_t n, int64_t base, int toff, int
         int prompt = (wr >> 30) & 1;
         int si_ssrb = sn1_ssrb[si];
         if (btp == 2) {
-          uint64_t g = (uint64_t)(base + i);
-          uint32_t c[4] = {(uint32_t)g, (uint32_t)(g >> 32), (uint32_t)r, 0u};
+          uint64_t g = (uint64_t)(base + i), q = g >> 2;
+          uint32_t c[4] = {(uint32_t)q, (uint32_t)(q >> 32), (uint32_t)r, 0u};
           philox(c, k0, k1);
-          uint32_t u = c[0];
+          uint32_t u = c[g & 3];
           N = 0;
           while ((int)N < ncdf - 1 && u >= cdf[N]) N++;
         }

@@ -57,6 +57,16 @@ def rel_err(got, ref, floor=1e-6):
     return float(np.max(np.abs(np.asarray(got, np.float64) - ref) / den))
 
 
+def worst_elem(got, ref, floor=1e-6):
+    """(index, |ref| / max|ref|, relative error) of the element that sets
+    rel_err: where a margin is decided, and at what signal level."""
+    ref = np.asarray(ref, np.float64)
+    den = np.maximum(np.abs(ref), floor * np.abs(ref).max())
+    e = np.abs(np.asarray(got, np.float64) - ref) / den
+    i = int(np.argmax(e))
+    return np.unravel_index(i, ref.shape), float(abs(ref.flat[i]) / np.abs(ref).max()), float(e.flat[i])
+
+
 def margin(name, err, bar):
     """Print a parity margin (measured error against its bar) and return
     whether it holds."""

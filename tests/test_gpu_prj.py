@@ -11,7 +11,7 @@ Every check prints its measured margin.
 import numpy as np
 import pytest
 
-from conftest import margin, phantom, rel_err
+from conftest import margin, phantom, rel_err, worst_elem
 
 pytestmark = pytest.mark.gpu
 
@@ -37,6 +37,10 @@ def img():
 
 
 def _ok(name, gpu, ref, bar=FP_RTOL):
+    gpu = gpu.cpu().numpy() if hasattr(gpu, 'cpu') else np.asarray(gpu)
+    idx, lvl, _ = worst_elem(gpu, ref)
+    print(f'  worst element of {name}: index {tuple(int(i) for i in idx)}, |ref| = {lvl:.3g} of max',
+          flush=True)
     return margin(name, rel_err(gpu, ref), bar)
 
 

@@ -93,6 +93,34 @@ def test_bootstrap_weights_statistics(params):
     assert abs(z0 / n - np.exp(-1)) < 0.005
 
 
+def test_bootstrap_counter_layout(params):
+    """Every event's weight, per replicate, from the KAT-pinned generator:
+    N = #{k < 39 : u >= cdf[k]} with u = output word (g mod 4) of
+    Philox4x32-10(counter (g // 4, 0, replicate, 0), key (seed lo, seed hi)),
+    g the global word index; counted when 0 < N < 32. Events sit in distinct
+    bins, so every bin holds one event's weight."""
+    s11 = params['axLUT']['sn1_sn11'].astype(np.int64)
+    seed, nrep, base, nev = (7 << 32) | 4242, 3, 1001, 61
+    addrs = [(si, (5 * si) % 252, (7 * si) % 344) for si in range(1, 4084, 4084 // nev)][:nev]
+    words = np.array([TAG | 0] + [(1 << 30) | _addr(*a) for a in addrs] + [TAG | 999],
+                     np.int64).astype(np.int32)
+    h = O.lm_hist(words, params, 0, 1, btp=2, btprt=1.0, seed=seed, nrep=nrep, base=base)
+    cdf = O.poisson_cdf(1.0, 40).astype(np.int64)
+    key = [seed & 0xffffffff, seed >> 32]
+    nonzero = 0
+    for r in range(nrep):
+        exp = np.zeros(837 * 86688, np.uint32)
+        for i, (si, a, w) in enumerate(addrs):
+            g = base + 1 + i
+            u = int(O.philox4x32([(g >> 2) & 0xffffffff, g >> 34, r, 0], key)[g & 3])
+            n = int(np.sum(u >= cdf[:39]))
+            if 0 < n < 32:
+                exp[s11[si] * 86688 + a * 344 + w] += n
+        assert np.array_equal(h['psino'][r], exp)
+        nonzero += int(np.count_nonzero(exp))
+    assert 0 < nonzero < nrep * nev     # some events drawn 0, some counted
+
+
 def _patch_stream(words_per_patch, fill, npatch):
     """Stream of npatch 8192-word patches: time tags at the first word of
     every patch (1 s apart), then `fill(k)` words (k = patch index)."""
