@@ -134,10 +134,10 @@ __device__ inline float rdlane(float v, int l) {
 // the staged chunk, LOR a is on plane ia and the next z boundary lies at
 // t = tb: it adds p1 = clamp(tb - s, 0, L) times plane ia and L - p1 times
 // plane ia + 1 (mirror: ib and ib - 1), and steps to the next plane when
-// tb <= s + L. tb is tracked in float relative to the chunk's first segment
-// start and recomputed exactly (FP64, from the plane index) at every chunk, so
-// positions along 67 cm rays keep 1e-6 cm resolution without FP64 in the
-// inner loop. Bins whose segments can hold two z boundaries (segment longer
+// tb <= s + L. tb is tracked in float relative to the start of the chunk's
+// middle segment and recomputed exactly (FP64, from the plane index) at every
+// chunk, so positions along 67 cm rays keep 1e-7 cm resolution without FP64
+// in the inner loop. Bins whose segments can hold two z boundaries (segment longer
 // than the boundary spacing) take a three-piece step. Pieces go into float
 // partial sums over SUBSUM segments folded into float totals (one long float
 // sum of nearly equal segment lengths would round in the same direction at
@@ -167,7 +167,7 @@ __device__ inline void fma_nb(float (&acc)[NB], float p, const float (&x)[NB]) {
 }
 
 struct FpLor {
-  float tbr;   // next z boundary of LOR a relative to the chunk base (float)
+  float tbr;   // next z boundary of LOR a relative to the chunk's origin (float)
   float invf;  // boundary spacing (float)
   int ao;      // staged-chunk offset of plane ia (LOR a), segment 0
   int bo;      // staged-chunk offset of plane ib - 2 (mirror b: ib = 2 r0 + 2 r1 - ia)
@@ -187,7 +187,7 @@ struct FpSum {
   }
 };
 
-// Walk the nch staged segments (SR: start - chunk base, length; padding
+// Walk the nch staged segments (SR: start - chunk origin, length; padding
 // entries (-1e30, 0) contribute nothing and never cross). TWO: segments may
 // hold two z boundaries; DIRECT: direct planes only.
 template <int NB, bool DIRECT, bool TWO>
@@ -324,7 +324,9 @@ __global__ __launch_bounds__(FP_THREADS, NB >= 4 ? 4 : 6) void k_fprj(GeomView g
 
   for (int c = 0; c < nseg; c += CH) {
     const int nch = nseg - c < CH ? nseg - c : CH;
-    const double sb = g.seg[beg + c].s;  // chunk base position
+    // positions are relative to the chunk's middle segment: float positions
+    // within +-8 segments carry half the rounding of ones within 16
+    const double sb = g.seg[beg + c + (nch >> 1)].s;
     __syncthreads();
     if (tid < CH) {
       const SegRec sg = g.seg[beg + (c + tid < nseg ? c + tid : c)];
@@ -337,7 +339,7 @@ __global__ __launch_bounds__(FP_THREADS, NB >= 4 ? 4 : 6) void k_fprj(GeomView g
       if (k < nseg && z < NZ) v = img[((int64_t)g.seg[beg + k].colz + z) * nbt + b0 + b];
       zs[((z + FP_GUARD) * PS + j) * NB + b] = v;
     }
-    // exact next-boundary positions relative to the chunk base
+    // exact next-boundary positions relative to the chunk's middle segment
 #pragma unroll
     for (int r = 0; r < FP_ROUNDS; r++) {
       const int ia = L[r].ao / (PS * NB) - FP_GUARD;
@@ -591,7 +593,9 @@ __global__ __launch_bounds__(fpg_threads<G>()) void k_fprj_g(GeomView g, const f
 
   for (int c = 0; c < nseg; c += CH) {
     const int nch = nseg - c < CH ? nseg - c : CH;
-    const double sb = g.seg[beg + c].s;  // chunk base position
+    // positions are relative to the chunk's middle segment: float positions
+    // within +-8 segments carry half the rounding of ones within 16
+    const double sb = g.seg[beg + c + (nch >> 1)].s;
     __syncthreads();
     if (tid < CH) {
       const SegRec sg = g.seg[beg + (c + tid < nseg ? c + tid : c)];

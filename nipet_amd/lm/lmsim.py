@@ -55,8 +55,13 @@ def synth_lm(scanner_params, duration_s, per_ms, weights=None, prompt_frac=0.85,
     mem, cnt = torch.from_numpy(mem).cuda(), torch.from_numpy(cnt).cuda()
     naw = len(tx['aw2ali'])
     if weights is not None:
-        cdf = torch.cumsum(weights.reshape(-1).to(torch.float64), 0)
-        cdf /= cdf[-1].clone()
+        # integer weights (2^32 for the largest): an integer scan is exact, so
+        # the stream is the same on every run (a float scan regroups its sums
+        # with the device's scheduling and moves an event now and then)
+        wf = weights.reshape(-1).to(torch.float64)
+        wi = torch.round(wf.clamp(min=0) * (2.0**32 / float(wf.max()))).to(torch.int64)
+        cdf = torch.cumsum(wi, 0).to(torch.float64)
+        cdf /= float(cdf[-1])
     nevents = int(nev.sum())
     for e0 in range(0, nevents, chunk):
         e = torch.arange(e0, min(nevents, e0 + chunk), device='cuda', dtype=torch.int64)
