@@ -63,6 +63,7 @@ constexpr int LM_TILE = 8192;
 static_assert(((NSN1_BINS_MAX + LM_BBINS - 1) >> LM_BSHIFT) < 65536 && LM_TILE <= 65536,
               "k_lm_bin_part packs a bucket index and a rank into 16 bits each");
 constexpr int32_t TMSK = 0x1fffffff;
+constexpr int64_t LM_SPAN_END = 1 << 20;  // words searched at each end for the first / last tag
 
 __host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
   const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
@@ -244,6 +245,56 @@ __device__ inline void boot_quad(uint32_t (&c)[4], uint64_t g, uint32_t r, uint3
   philox4x32_10(c, k0, k1);
 }
 
+// Head curves and centre-of-mass sums, one replicate: each lane keeps running
+// sums for one second (its words are 64 apart, so a wave's 2,048 words rarely
+// span two seconds) and adds them when its second changes and after the walk.
+// Per-lane atomics on one LDS word serialise 64 deep, and a butterfly per word
+// row waited on the LDS crossbar; this adds once per lane and second.
+struct LmHead {
+  int k = INT32_MIN;  // second - window base of the sums (none yet)
+  uint32_t p = 0, d = 0;
+  unsigned long long z = 0;
+};
+
+__device__ inline void lm_head_flush(LmHead &h, int itag0, uint32_t *hcp, uint32_t *hcd,
+                                     unsigned long long *mzr, unsigned long long *mzm,
+                                     const LmDev &p) {
+  if (h.p | h.d) {
+    const int it = itag0 + h.k;
+    if (h.k >= 0 && h.k < LM_HW) {
+      if (h.p) {
+        atomicAdd(&hcp[h.k], h.p);
+        atomicAdd(&mzr[h.k], h.z);
+        atomicAdd(&mzm[h.k], (unsigned long long)h.p);
+      }
+      if (h.d) atomicAdd(&hcd[h.k], h.d);
+    } else {
+      if (h.p) {
+        atomicAdd(p.phc + it, h.p);
+        atomicAdd(p.zR + it, h.z);
+        atomicAdd(p.zM + it, (unsigned long long)h.p);
+      }
+      if (h.d) atomicAdd(p.dhc + it, h.d);
+    }
+  }
+  h.p = h.d = 0;
+  h.z = 0;
+}
+
+// one counted event: prompt weight hp with SSRB plane hz, or delayed weight hd
+__device__ inline void lm_head_add(LmHead &h, uint32_t hp, uint32_t hz, uint32_t hd, int k,
+                                   int itag0, uint32_t *hcp, uint32_t *hcd,
+                                   unsigned long long *mzr, unsigned long long *mzm,
+                                   const LmDev &p) {
+  if (k != h.k) {
+    lm_head_flush(h, itag0, hcp, hcd, mzr, mzm, p);
+    h.k = k;
+  }
+  h.p += hp;
+  h.d += hd;
+  h.z += hp ? hz : 0u;
+}
+
 // REP = false: one event per lane (nrep == 1).
 // REP = true: the wave walks its events one by one, lane r = replicate r.
 // NP: non-parametric bootstrap (btp = 1): the block's event words are
@@ -363,6 +414,7 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
   // REP: the last generator call of the wave (its word quad, the 4 draws)
   uint64_t bq = ~0ull;
   uint32_t bc[4] = {0u, 0u, 0u, 0u};
+  LmHead hs;  // one replicate: this lane's head-curve sums
   for (int j = 0; j < LM_PER_LANE; j++) {
     const int32_t word = wd[j];
     const int64_t gi = w0 + j * 64 + lane;
@@ -385,66 +437,66 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
       }
     }
     if (!REP) {
-      if (!ev) continue;
-      uint32_t N = 1;
-      int32_t wv = word;
-      if constexpr (NP) {
-        uint64_t g = (uint64_t)(p.base + gi);
-        uint32_t cc[4] = {(uint32_t)g, (uint32_t)(g >> 32), (uint32_t)p.rep0, 1u};
-        philox4x32_10(cc, p.k0, p.k1);
-        wv = evw[(int)(((uint64_t)cc[0] * (uint32_t)nev) >> 32)];
-        if ((wv & 0x3fffffff) >= NSN1 * NSBINANG) {
-          atomicAdd(p.bad, 1u);
-          continue;
+      // this lane's head-curve adds: counted prompt weight and its SSRB plane,
+      // counted delayed weight
+      uint32_t hp = 0, hz = 0, hd = 0;
+      if (ev) {
+        uint32_t N = 1;
+        int32_t wv = word;
+        bool drawn = true;
+        if constexpr (NP) {
+          uint64_t g = (uint64_t)(p.base + gi);
+          uint32_t cc[4] = {(uint32_t)g, (uint32_t)(g >> 32), (uint32_t)p.rep0, 1u};
+          philox4x32_10(cc, p.k0, p.k1);
+          wv = evw[(int)(((uint64_t)cc[0] * (uint32_t)nev) >> 32)];
+          if ((wv & 0x3fffffff) >= NSN1 * NSBINANG) {
+            atomicAdd(p.bad, 1u);
+            drawn = false;
+          }
+        } else if (p.btp == 2) {
+          const uint64_t g = (uint64_t)(p.base + gi);
+          uint32_t cc[4];
+          boot_quad(cc, g, (uint32_t)p.rep0, p.k0, p.k1);
+          N = poisson_n(boot_pick(cc, g), c8, cdf);
         }
-      } else if (p.btp == 2) {
-        const uint64_t g = (uint64_t)(p.base + gi);
-        uint32_t cc[4];
-        boot_quad(cc, g, (uint32_t)p.rep0, p.k0, p.k1);
-        N = poisson_n(boot_pick(cc, g), c8, cdf);
-      }
-      const bool okN = N > 0 && N < 32;
-      if constexpr (!BIN) {
-        if (!okN) continue;
-      }
-      const int32_t val = wv & 0x3fffffff;
-      const int si = val / NSBINANG, aw = val - si * NSBINANG;
-      const int a = aw / NSBINS, w = aw - a * NSBINS;
-      const int32_t lu = lut[si];
-      const int s11 = lu >> 16, sr = lu & 0xffff;
-      const int64_t addr = p.span == 11 ? (int64_t)s11 * NSBINANG + aw
-                                        : (p.span == 1 ? (int64_t)val : (int64_t)sr * NSBINANG + aw);
-      if constexpr (BIN) {
-        // every counted event takes a slot of its coarse run in LDS (weight 0
-        // when not counted); the runs go out after the walk
-        const uint32_t s = atomicAdd(&crank[(int)(addr >> LM_CSHIFT)], 1u);
-        if (s < LM_BLOCK)  // the runs were counted for exactly these events
-          stg[s] = (uint32_t)(addr & ((1 << LM_CSHIFT) - 1)) |
-                   ((uint32_t)((wv >> 30) & 1) << LM_CSHIFT) | ((okN ? N : 0u) << (LM_CSHIFT + 1));
-        if (!okN) continue;
-      }
-      const int k = itag - itag0;
-      const bool inwin = k >= 0 && k < LM_HW;
-      if ((wv >> 30) & 1) {
-        if constexpr (!BIN) atomicAdd(p.psino + addr, N);
-        if (inwin) atomicAdd(&hcp[k], N); else atomicAdd(p.phc + itag, N);
-        if (inwin) {
-          atomicAdd(&mzr[k], (unsigned long long)sr);
-          atomicAdd(&mzm[k], (unsigned long long)N);
-        } else {
-          atomicAdd(p.zR + itag, (unsigned long long)sr);
-          atomicAdd(p.zM + itag, (unsigned long long)N);
+        const bool okN = N > 0 && N < 32;
+        if (drawn && (BIN || okN)) {
+          const int32_t val = wv & 0x3fffffff;
+          const int si = val / NSBINANG, aw = val - si * NSBINANG;
+          const int a = aw / NSBINS, w = aw - a * NSBINS;
+          const int32_t lu = lut[si];
+          const int s11 = lu >> 16, sr = lu & 0xffff;
+          const int64_t addr = p.span == 11 ? (int64_t)s11 * NSBINANG + aw
+                                            : (p.span == 1 ? (int64_t)val : (int64_t)sr * NSBINANG + aw);
+          if constexpr (BIN) {
+            // every counted event takes a slot of its coarse run in LDS (weight
+            // 0 when not counted); the runs go out after the walk
+            const uint32_t s = atomicAdd(&crank[(int)(addr >> LM_CSHIFT)], 1u);
+            if (s < LM_BLOCK)  // the runs were counted for exactly these events
+              stg[s] = (uint32_t)(addr & ((1 << LM_CSHIFT) - 1)) |
+                       ((uint32_t)((wv >> 30) & 1) << LM_CSHIFT) | ((okN ? N : 0u) << (LM_CSHIFT + 1));
+          }
+          if (okN) {
+            if ((wv >> 30) & 1) {
+              if constexpr (!BIN) atomicAdd(p.psino + addr, N);
+              hp = N;
+              hz = (uint32_t)sr;
+              if ((a == 0 || a == 126) && itag < MXNITAG)
+                atomicAdd(p.pvs + (int64_t)(itag >> VTIME) * SEG0 * NSBINS + sr * NSBINS + w,
+                          N << (8 * (a == 126)));
+            } else {
+              if constexpr (!BIN) atomicAdd(p.dsino + addr, N);
+              hd = N;
+              if constexpr (!BIN) {  // binned sessions: k_lm_fan
+                short2 rr = p.rno[si], cr = p.s2cF[a + NSANGLES * w];
+                atomicAdd(p.fan + NCRS * rr.x + cr.x, N);
+                atomicAdd(p.fan + NCRS * rr.y + cr.y, N);
+              }
+            }
+          }
         }
-        if ((a == 0 || a == 126) && itag < MXNITAG)
-          atomicAdd(p.pvs + (int64_t)(itag >> VTIME) * SEG0 * NSBINS + sr * NSBINS + w,
-                    N << (8 * (a == 126)));
-      } else {
-        if constexpr (!BIN) atomicAdd(p.dsino + addr, N);
-        if (inwin) atomicAdd(&hcd[k], N); else atomicAdd(p.dhc + itag, N);
-        short2 rr = p.rno[si], cr = p.s2cF[a + NSANGLES * w];
-        atomicAdd(p.fan + NCRS * rr.x + cr.x, N);
-        atomicAdd(p.fan + NCRS * rr.y + cr.y, N);
       }
+      if (hp | hd) lm_head_add(hs, hp, hz, hd, itag - itag0, itag0, hcp, hcd, mzr, mzm, p);
     } else if constexpr (NP) {
       // lane r draws replicate r's word: bins differ between lanes
       unsigned long long evm = __ballot(ev);
@@ -570,20 +622,17 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
       }
     }
   }
+  if (!REP) lm_head_flush(hs, itag0, hcp, hcd, mzr, mzm, p);
   __syncthreads();
   if constexpr (BIN) {
-    // crank now holds the end of every run; slot s belongs to the first run
-    // that ends after it. Consecutive lanes write consecutive words of a run.
-    const int ncb = p.ncb;
-    const uint32_t nst = crank[ncb - 1];
-    for (uint32_t s = tid; s < nst; s += LM_THREADS) {
-      int lo = 0, hi = ncb - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (crank[mid] > s) hi = mid; else lo = mid + 1;
+    // crank now holds the end of every run: wave k writes runs k, k + 4, ...,
+    // consecutive lanes on consecutive slots
+    for (int cb = wave; cb < p.ncb; cb += LM_WAVES) {
+      const uint32_t e = crank[cb], gb = cbase[cb];
+      for (uint32_t sl = (cb ? crank[cb - 1] : 0u) + lane; sl < e; sl += 64) {
+        const uint32_t pos = gb + sl;
+        if ((int64_t)pos < p.stage_cap) p.stage[pos] = stg[sl];
       }
-      const uint32_t pos = cbase[lo] + s;
-      if ((int64_t)pos < p.stage_cap) p.stage[pos] = stg[s];
     }
   }
   // flush the LDS windows
@@ -649,6 +698,84 @@ __global__ void k_lm_ssrb(const uint32_t *psino, int nrep, const int32_t *s_off,
       acc += psino[((int64_t)s_list[k] * NSBINANG + aw) * nrep];
     ssr[i] = acc;
   }
+}
+
+// Delayed fan-sums of one-replicate sessions (the sessions whose sinogram
+// counts are LDS-privatised). In the event kernel each delayed event's two
+// crystal ends were global atomics into the 64 x 504 fan-sums (about 2 ms of
+// device atomics for a 600 s stream). Here each workgroup (one per CU: the
+// fan-sums take 126 KB of LDS) walks LM_FAN_SUB word blocks at a time, with
+// the same tag walk and tests as k_lm_hist, counts the crystal ends in LDS and
+// writes its copy to part[blockIdx.x]; k_lm_fan_sum adds the copies to the
+// output.
+constexpr int LM_FAN = 64 * NCRS;
+constexpr int LM_FAN_WG = 256;
+constexpr int LM_FAN_SUB = 4;
+
+__global__ __launch_bounds__(LM_THREADS * LM_FAN_SUB) void k_lm_fan(LmDev p, int64_t nblk,
+                                                                   uint32_t *part) {
+  __shared__ uint32_t fan[LM_FAN];
+  __shared__ uint32_t cdf[LM_NCDF];
+  __shared__ int32_t wave_last[LM_FAN_SUB][LM_WAVES];
+  const int tid = threadIdx.x, lane = tid & 63, sub = tid / LM_THREADS;
+  const int wave = (tid / 64) % LM_WAVES;
+  for (int i = tid; i < LM_FAN; i += LM_THREADS * LM_FAN_SUB) fan[i] = 0;
+  if (tid < LM_NCDF) cdf[tid] = p.cdf[tid];
+  __syncthreads();
+  uint32_t c8[LM_FASTK];
+#pragma unroll
+  for (int k = 0; k < LM_FASTK; k++) c8[k] = __builtin_amdgcn_readfirstlane(cdf[k]);
+  const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1);
+  const int64_t step = (int64_t)gridDim.x * LM_FAN_SUB;
+  const int64_t rounds = (nblk + step - 1) / step;  // the same for every thread
+  for (int64_t r = 0; r < rounds; r++) {
+    const int64_t blk = r * step + (int64_t)blockIdx.x * LM_FAN_SUB + sub;
+    const bool on = blk < nblk;  // uniform over the sub-block
+    const int64_t w0 = blk * LM_BLOCK + (int64_t)wave * (LM_PER_LANE * 64);
+    int32_t wd[LM_PER_LANE];
+    __syncthreads();  // wave_last of the previous round read
+    int32_t wl = -1;
+    if (on) wl = lm_load(p, w0, lane, wd);
+    if (lane == 0) wave_last[sub][wave] = wl;
+    __syncthreads();
+    if (!on) continue;
+    int32_t c = p.blk_carry[blk];
+    for (int k = 0; k < wave; k++)
+      if (wave_last[sub][k] >= 0) c = wave_last[sub][k];
+    for (int j = 0; j < LM_PER_LANE; j++) {
+      const int32_t word = wd[j];
+      const int32_t ms = lm_tag_at(word, c, le);
+      const int itag = (ms < 0 ? 0 : ms - p.toff) / ITIME;
+      const bool del = word > 0 && !((word >> 30) & 1) && itag >= p.tstart && itag < p.tstop &&
+                       (word & 0x3fffffff) < NSN1 * NSBINANG;
+      uint32_t N = 1;
+      if (p.btp == 2) {  // the event's weight, as k_lm_hist draws it
+        const uint64_t g = (uint64_t)(p.base + w0 + j * 64 + lane);
+        uint32_t cc[4];
+        boot_quad(cc, g, (uint32_t)p.rep0, p.k0, p.k1);
+        N = poisson_n(boot_pick(cc, g), c8, cdf);
+      }
+      if (del && N > 0 && N < 32) {
+        const int32_t val = word & 0x3fffffff;
+        const int si = val / NSBINANG, aw = val - si * NSBINANG;
+        const int a = aw / NSBINS, w = aw - a * NSBINS;
+        const short2 rr = p.rno[si], cr = p.s2cF[a + NSANGLES * w];
+        atomicAdd(&fan[NCRS * rr.x + cr.x], N);
+        atomicAdd(&fan[NCRS * rr.y + cr.y], N);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < LM_FAN; i += LM_THREADS * LM_FAN_SUB)
+    part[(int64_t)blockIdx.x * LM_FAN + i] = fan[i];
+}
+
+__global__ void k_lm_fan_sum(const uint32_t *__restrict__ part, int nwg, uint32_t *fan) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= LM_FAN) return;
+  uint32_t acc = 0;
+  for (int b = 0; b < nwg; b++) acc += part[(int64_t)b * LM_FAN + i];
+  fan[i] += acc;
 }
 
 __global__ void k_fill_u64(unsigned long long *p, unsigned long long v, int64_t n) {
@@ -885,13 +1012,14 @@ struct NipetLm {
   uint32_t *gccnt = nullptr, *cstart = nullptr, *ccur = nullptr;
   uint32_t *stage = nullptr, *stage2 = nullptr;  // coarse order, bucket order
   int64_t stage_cap = 0;
+  uint32_t *fan_part = nullptr;  // [LM_FAN_WG][64 x 504] fan-sum copies
 };
 
 static void lm_free(NipetLm *h) {
   void *ps[] = {h->lut, h->rno, h->s2cF, h->cdf, h->carry, h->blk_last, h->blk_carry,
                 h->first_nz, h->sgl_idx, h->sgl_word, h->sgl_ms, h->sgl_count, h->bad,
                 h->ssrb_off, h->ssrb_list, h->gcnt, h->bstart, h->bcur, h->gccnt,
-                h->cstart, h->ccur, h->stage, h->stage2};
+                h->cstart, h->ccur, h->stage, h->stage2, h->fan_part};
   for (void *q : ps) (void)hipFree(q);
 }
 
@@ -974,6 +1102,7 @@ static int lm_build(const NipetLmConfig *c, NipetLm *h) {
     NIPET_HIP(hipMalloc(&h->gccnt, (size_t)h->ncb * sizeof(uint32_t)));
     NIPET_HIP(hipMalloc(&h->cstart, (size_t)(h->ncb + 1) * sizeof(uint32_t)));
     NIPET_HIP(hipMalloc(&h->ccur, (size_t)h->ncb * sizeof(uint32_t)));
+    NIPET_HIP(hipMalloc(&h->fan_part, (size_t)LM_FAN_WG * LM_FAN * sizeof(uint32_t)));
   }
   NIPET_HIP(hipMemcpy(h->lut, lut.data(), NSN1 * sizeof(int32_t), hipMemcpyHostToDevice));
   NIPET_HIP(hipMemcpy(h->rno, c->sn1_rno, NSN1 * sizeof(short2), hipMemcpyHostToDevice));
@@ -992,15 +1121,22 @@ static int lm_build(const NipetLmConfig *c, NipetLm *h) {
 // ---- time-tag index of a device word stream: what getLMinfo finds by scanning
 // the file (first / last time tag, word positions of given times,
 // lm/src/lmaux.cu:17-200), for a stream already in device memory ----
-__global__ void k_lm_tag_span(const int32_t *__restrict__ w, int64_t n,
-                              unsigned long long *span) {
-  // span[0] = smallest index of a time-tag word, span[1] = largest index + 1
+// span[0] = smallest index of a time-tag word, span[1] = largest index + 1,
+// over the indices [a0, a1) and [b0, b1). With skip set (a scan of the middle
+// [a0, a1) after the two ends), a launch returns at once when the first tag lay
+// before a0 and the last at or after a1.
+__global__ void k_lm_tag_span(const int32_t *__restrict__ w, int64_t n, int64_t a0, int64_t a1,
+                              int64_t b0, int64_t b1, int skip, unsigned long long *span) {
+  if (skip && span[0] < (unsigned long long)a0 && span[1] > (unsigned long long)a1)
+    return;  // grid-uniform
   unsigned long long lo = (unsigned long long)n, hi = 0ull;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
+  const int64_t na = a1 - a0, nt = na + (b1 - b0);
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nt;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t < na ? a0 + t : b0 + (t - na);
     if (is_tag(w[i])) {
       lo = lo < (unsigned long long)i ? lo : (unsigned long long)i;
-      hi = (unsigned long long)(i + 1);
+      hi = hi > (unsigned long long)(i + 1) ? hi : (unsigned long long)(i + 1);
     }
   }
   for (int o = 32; o > 0; o >>= 1) {
@@ -1211,6 +1347,9 @@ int nipet_lm_chunk(NipetLm *h, const int32_t *words, int64_t n, const NipetLmOut
                                                              h->bcur, h->stage2);
     k_lm_bin_apply<<<(unsigned)h->nbk, 1024, 0, st>>>(h->stage2, h->bstart, h->nbins, o->psino,
                                                        o->dsino);
+    const int nwg = (int)std::min<int64_t>((nblk + LM_FAN_SUB - 1) / LM_FAN_SUB, LM_FAN_WG);
+    k_lm_fan<<<(unsigned)nwg, LM_THREADS * LM_FAN_SUB, 0, st>>>(p, nblk, h->fan_part);
+    k_lm_fan_sum<<<(LM_FAN + 255) / 256, 256, 0, st>>>(h->fan_part, nwg, o->fan);
   } else if (c.nrep == 1) {
     k_lm_hist<false, false><<<(unsigned)nblk, LM_THREADS, 0, st>>>(p);
   } else {
@@ -1262,8 +1401,15 @@ int nipet_lm_tag_span(const int32_t *words, int64_t n, int64_t *span, void *stre
   k_fill_u64<<<1, 64, 0, st>>>(sp, (unsigned long long)n, 1);
   k_fill_u64<<<1, 64, 0, st>>>(sp + 1, 0ull, 1);
   if (n > 0) {
-    const int64_t nb = std::min<int64_t>((n + 255) / 256, 8192);
-    k_lm_tag_span<<<(unsigned)nb, 256, 0, st>>>(words, n, sp);
+    // the first and last LM_SPAN_END words (a stream's first and last tags
+    // lie there), then the rest only when an end held no tag
+    const int64_t e = std::min<int64_t>(LM_SPAN_END, n), b = std::max<int64_t>(e, n - LM_SPAN_END);
+    const int64_t ne = e + (n - b);
+    k_lm_tag_span<<<(unsigned)std::min<int64_t>((ne + 255) / 256, 2048), 256, 0, st>>>(
+        words, n, 0, e, b, n, 0, sp);
+    if (b > e)
+      k_lm_tag_span<<<(unsigned)std::min<int64_t>((b - e + 255) / 256, 8192), 256, 0, st>>>(
+          words, n, e, b, b, b, 1, sp);
   }
   NIPET_HIP(hipGetLastError());
   return NIPET_OK;

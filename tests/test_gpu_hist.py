@@ -196,6 +196,22 @@ def test_time_index_on_device(torch, stream):
     assert second_index(events, tf, 3).tolist() == [199] * 4
 
 
+def test_tag_span_long_streams(torch):
+    """First / last tag of streams longer than the two end windows (2^20 words
+    each): tags in both ends, only in the middle, only at one end, and a tag
+    in the middle beside end tags (never chosen)."""
+    from nipet_amd.lm.mmrhist import tag_range
+    n = 3_500_000
+    cases = [[5, 1_700_000, n - 3], [1_500_000, 2_000_000], [17, 900_000, 1_600_000],
+             [1_600_000, n - 1], [0, n - 1]]
+    for pos in cases:
+        w = np.full(n, (1 << 30) | 12345, np.int32)
+        for k, i in enumerate(pos):
+            w[i] = np.int32(-(1 << 31) | (1000 + 7 * k))
+        got = tag_range(torch.from_numpy(w).cuda())
+        assert got == (1000, 1000 + 7 * (len(pos) - 1)), (pos, got)
+
+
 def test_rep0_selects_replicate(torch, params, stream):
     """A session with nrep = 1, rep0 = 2 histograms exactly replicate 2 of an
     nrep = 3 session (oracle), so replicate sets can be split across sessions."""
