@@ -868,19 +868,27 @@ __global__ __launch_bounds__(1024) void k_lm_bin_scan(const uint32_t *gcnt, int 
 }
 
 // Pass 3: tiles of LM_TILE events of the coarse stage (s1, coarse buckets
-// [cstart[c], cstart[c + 1])) ranked in LDS by bucket and written to the
+// [cstart[c], cstart[c + 1])) sorted by bucket in LDS and written to the
 // buckets' ranges of s2 (cursors fcur) in runs, as (bin within bucket) |
 // prompt << LM_BSHIFT | N << (LM_BSHIFT + 1). A thread keeps its 32 events
-// and their ranks in registers; LDS holds only the tile's bucket counters.
+// and their buckets in registers; per window of at most LM_PART_W buckets
+// (one window unless the tile spans many sparse coarse buckets) the events
+// are ranked with LDS atomics, one cursor atomic per bucket reserves each
+// run, the events are placed in LDS by run and wave k writes runs k, k + 4,
+// ... with consecutive lanes on consecutive slots.
+constexpr int LM_PART_W = 2048;
+
 __global__ __launch_bounds__(LM_THREADS) void k_lm_bin_part(const uint32_t *__restrict__ s1,
                                                             const uint32_t *__restrict__ cstart,
                                                             int ncb, int nbk, uint32_t *fcur,
                                                             uint32_t *s2) {
-  extern __shared__ uint32_t fc[];  // bucket counters of the tile's window, then run bases
+  __shared__ uint32_t cnt[LM_PART_W + 1];  // counts, then run starts (cnt[nw] = total)
+  __shared__ uint32_t gb[LM_PART_W];       // global slot of the run's local slot 0
+  __shared__ uint32_t stg[LM_TILE];
   __shared__ uint32_t cs[LM_MAXCB + 1];
   __shared__ int win[2];  // first and last coarse bucket of the tile
   constexpr int PER = LM_TILE / LM_THREADS;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t total = cstart[ncb];
   const int64_t t0 = (int64_t)blockIdx.x * LM_TILE;
   if (t0 >= (int64_t)total) return;  // block-uniform, before any barrier
@@ -908,34 +916,63 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_bin_part(const uint32_t *__re
   }
   __syncthreads();
   const int c0 = win[0], c1 = win[1], f0 = c0 * LM_FPC;
-  const int f1 = (c1 + 1) * LM_FPC < nbk ? (c1 + 1) * LM_FPC : nbk;
-  for (int i = tid; i < f1 - f0; i += LM_THREADS) fc[i] = 0;
-  __syncthreads();
-  // rank within the bucket (< LM_TILE) | bucket within the window << 16
+  const int nf = ((c1 + 1) * LM_FPC < nbk ? (c1 + 1) * LM_FPC : nbk) - f0;
+  // bucket within the tile's window << 16 | rank within the bucket
   uint32_t rf[PER];
 #pragma unroll
   for (int k = 0; k < PER; k++) {
     const int i = k * LM_THREADS + tid;
-    rf[k] = 0;
-    if (i < nt) {
-      const int cb = coarse((uint32_t)(t0 + i), c0, c1);
-      const int f = cb * LM_FPC + (int)((v[k] & ((1u << LM_CSHIFT) - 1)) >> LM_BSHIFT) - f0;
-      rf[k] = atomicAdd(&fc[f], 1u) | ((uint32_t)f << 16);
-    }
+    const int cb = i < nt ? coarse((uint32_t)(t0 + i), c0, c1) : c0;
+    rf[k] = (uint32_t)(cb * LM_FPC + (int)((v[k] & ((1u << LM_CSHIFT) - 1)) >> LM_BSHIFT) - f0)
+            << 16;
   }
-  __syncthreads();
-  for (int i = tid; i < f1 - f0; i += LM_THREADS) {
-    const uint32_t n = fc[i];
-    fc[i] = n ? atomicAdd(fcur + f0 + i, n) : 0u;
-  }
-  __syncthreads();
+  for (int w0 = 0; w0 < nf; w0 += LM_PART_W) {  // block-uniform
+    const int nw = nf - w0 < LM_PART_W ? nf - w0 : LM_PART_W;
+    __syncthreads();  // the previous window's runs written out
+    for (int i = tid; i < nw; i += LM_THREADS) cnt[i] = 0;
+    __syncthreads();
 #pragma unroll
-  for (int k = 0; k < PER; k++) {
-    const int i = k * LM_THREADS + tid;
-    if (i < nt) {
-      const uint32_t pos = fc[rf[k] >> 16] + (rf[k] & 0xffffu);
-      if (pos < total)  // the ranges were counted for exactly these events
-        s2[pos] = (v[k] & (LM_BBINS - 1)) | ((v[k] >> LM_CSHIFT) << LM_BSHIFT);
+    for (int k = 0; k < PER; k++) {
+      const int f = (int)(rf[k] >> 16) - w0;
+      if (k * LM_THREADS + tid < nt && f >= 0 && f < nw)
+        rf[k] = (rf[k] & 0xffff0000u) | atomicAdd(&cnt[f], 1u);
+    }
+    __syncthreads();
+    if (wave == 0) {  // run starts (exclusive scan) and one cursor atomic per run
+      uint32_t run = 0;
+      for (int i0 = 0; i0 < nw; i0 += 64) {
+        const int i = i0 + lane;
+        const uint32_t n = i < nw ? cnt[i] : 0u;
+        uint32_t x = n;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const uint32_t y = __shfl_up(x, o);
+          if (lane >= o) x += y;
+        }
+        const uint32_t ex = run + x - n;
+        if (i < nw) {
+          gb[i] = (n ? atomicAdd(fcur + f0 + w0 + i, n) : 0u) - ex;
+          cnt[i] = ex;
+        }
+        run += __shfl(x, 63);
+      }
+      if (lane == 0) cnt[nw] = run;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int f = (int)(rf[k] >> 16) - w0;
+      if (k * LM_THREADS + tid < nt && f >= 0 && f < nw)
+        stg[cnt[f] + (rf[k] & 0xffffu)] =
+            (v[k] & (LM_BBINS - 1)) | ((v[k] >> LM_CSHIFT) << LM_BSHIFT);
+    }
+    __syncthreads();
+    for (int f = wave; f < nw; f += LM_WAVES) {
+      const uint32_t e = cnt[f + 1], base = gb[f];
+      for (uint32_t sl = cnt[f] + lane; sl < e; sl += 64) {
+        const uint32_t pos = base + sl;
+        if (pos < total) s2[pos] = stg[sl];  // the ranges were counted for exactly these events
+      }
     }
   }
 }
@@ -1342,9 +1379,8 @@ int nipet_lm_chunk(NipetLm *h, const int32_t *words, int64_t n, const NipetLmOut
     k_lm_bin_scan<<<1, 1024, 0, st>>>(h->gcnt, h->nbk, h->bstart, h->bcur);
     k_lm_bin_scan<<<1, 1024, 0, st>>>(h->gccnt, h->ncb, h->cstart, h->ccur);
     k_lm_hist<false, false, true><<<(unsigned)nblk, LM_THREADS, 0, st>>>(p);
-    k_lm_bin_part<<<(unsigned)((n + LM_TILE - 1) / LM_TILE), LM_THREADS,
-                    (size_t)h->nbk * sizeof(uint32_t), st>>>(h->stage, h->cstart, h->ncb, h->nbk,
-                                                             h->bcur, h->stage2);
+    k_lm_bin_part<<<(unsigned)((n + LM_TILE - 1) / LM_TILE), LM_THREADS, 0, st>>>(
+        h->stage, h->cstart, h->ncb, h->nbk, h->bcur, h->stage2);
     k_lm_bin_apply<<<(unsigned)h->nbk, 1024, 0, st>>>(h->stage2, h->bstart, h->nbins, o->psino,
                                                        o->dsino);
     const int nwg = (int)std::min<int64_t>((nblk + LM_FAN_SUB - 1) / LM_FAN_SUB, LM_FAN_WG);

@@ -104,6 +104,20 @@ def test_hist_span1_and_ssrb(torch, params, stream, span):
         assert g['psino'].shape[0] == (4084 if span == 1 else 127) * 86688
 
 
+def test_hist_sparse_span1(torch, params, stream):
+    """A short span-1 session: one tile of staged events spans all 338 coarse
+    buckets (21,609 buckets, more than one LDS window of the partition pass);
+    every output bit-exact against the oracle."""
+    from oracle import oracle as O
+    toff, nitag = _toff_nitag(stream)
+    w = stream[:20_000]
+    for btp in (0, 2):
+        h, g = _gpu(torch, params, w, toff, nitag, span=1, btp=btp, seed=8)
+        o = O.lm_hist(w, params, toff, nitag, span=1, btp=btp, seed=8)
+        _compare(g, o, 1)
+        assert o['psino'].sum() > 10_000
+
+
 def test_hist_chunked_frame_window(torch, params, stream):
     from oracle import oracle as O
     toff, nitag = _toff_nitag(stream)
