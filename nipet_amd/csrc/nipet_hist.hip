@@ -60,6 +60,7 @@ constexpr int LM_CSHIFT = 20;
 constexpr int LM_FPC = 1 << (LM_CSHIFT - LM_BSHIFT);  // buckets per coarse bucket
 constexpr int LM_MAXCB = (NSN1_BINS_MAX + (1 << LM_CSHIFT) - 1) >> LM_CSHIFT;
 constexpr int LM_TILE = 8192;
+static_assert(LM_MAXCB < 0xffff, "k_lm_hist keeps a coarse bucket in 16 bits");
 static_assert(((NSN1_BINS_MAX + LM_BBINS - 1) >> LM_BSHIFT) < 65536 && LM_TILE <= 65536,
               "k_lm_bin_part packs a bucket index and a rank into 16 bits each");
 constexpr int32_t TMSK = 0x1fffffff;
@@ -375,46 +376,17 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
   }
 
   const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1);
-  if constexpr (BIN) {
-    // the block's counted events per coarse bucket (the same walk and test as
-    // below), then one run of slots per coarse bucket: crank becomes the
-    // run's first slot in stg (exclusive scan by wave 0), cbase the run's
-    // global slot minus that local start
-    int32_t c1 = c;
-    for (int j = 0; j < LM_PER_LANE; j++) {
-      const int32_t word = wd[j];
-      const int32_t ms = lm_tag_at(word, c1, le);
-      const int itag = (ms < 0 ? 0 : ms - p.toff) / ITIME;
-      if (word > 0 && itag >= p.tstart && itag < p.tstop &&
-          (word & 0x3fffffff) < NSN1 * NSBINANG)
-        atomicAdd(&crank[lm_bin(p.span, word & 0x3fffffff, lut) >> LM_CSHIFT], 1u);
-    }
-    __syncthreads();
-    if (wave == 0) {
-      uint32_t run = 0;
-      for (int i0 = 0; i0 < p.ncb; i0 += 64) {
-        const int i = i0 + lane;
-        const uint32_t nc = i < p.ncb ? crank[i] : 0u;
-        uint32_t x = nc;  // inclusive scan over the wave
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-          const uint32_t y = __shfl_up(x, o);
-          if (lane >= o) x += y;
-        }
-        const uint32_t ex = run + x - nc;
-        if (i < p.ncb) {
-          cbase[i] = (nc ? atomicAdd(p.ccur + i, nc) : 0u) - ex;
-          crank[i] = ex;
-        }
-        run += __shfl(x, 63);
-      }
-    }
-    __syncthreads();
-  }
   // REP: the last generator call of the wave (its word quad, the 4 draws)
   uint64_t bq = ~0ull;
   uint32_t bc[4] = {0u, 0u, 0u, 0u};
   LmHead hs;  // one replicate: this lane's head-curve sums
+  // BIN: each word's staged value and coarse bucket (16 bits each, two per
+  // register; 0xffff: not counted), kept until the block's runs are laid out
+  uint32_t bpay[BIN ? LM_PER_LANE : 1], bcb[BIN ? LM_PER_LANE / 2 : 1];
+  if constexpr (BIN) {
+#pragma unroll
+    for (int j = 0; j < LM_PER_LANE / 2; j++) bcb[j] = ~0u;
+  }
   for (int j = 0; j < LM_PER_LANE; j++) {
     const int32_t word = wd[j];
     const int64_t gi = w0 + j * 64 + lane;
@@ -469,12 +441,14 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
           const int64_t addr = p.span == 11 ? (int64_t)s11 * NSBINANG + aw
                                             : (p.span == 1 ? (int64_t)val : (int64_t)sr * NSBINANG + aw);
           if constexpr (BIN) {
-            // every counted event takes a slot of its coarse run in LDS (weight
-            // 0 when not counted); the runs go out after the walk
-            const uint32_t s = atomicAdd(&crank[(int)(addr >> LM_CSHIFT)], 1u);
-            if (s < LM_BLOCK)  // the runs were counted for exactly these events
-              stg[s] = (uint32_t)(addr & ((1 << LM_CSHIFT) - 1)) |
-                       ((uint32_t)((wv >> 30) & 1) << LM_CSHIFT) | ((okN ? N : 0u) << (LM_CSHIFT + 1));
+            // every counted event takes a slot of its coarse run (weight 0
+            // when not counted): counted here, placed after the walk
+            const uint32_t cb = (uint32_t)(addr >> LM_CSHIFT);
+            atomicAdd(&crank[cb], 1u);
+            const int sh = 16 * (j & 1);
+            bcb[j >> 1] = (bcb[j >> 1] & ~(0xffffu << sh)) | (cb << sh);
+            bpay[j] = (uint32_t)(addr & ((1 << LM_CSHIFT) - 1)) |
+                      ((uint32_t)((wv >> 30) & 1) << LM_CSHIFT) | ((okN ? N : 0u) << (LM_CSHIFT + 1));
           }
           if (okN) {
             if ((wv >> 30) & 1) {
@@ -623,6 +597,40 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_hist(LmDev p) {
     }
   }
   if (!REP) lm_head_flush(hs, itag0, hcp, hcd, mzr, mzm, p);
+  if constexpr (BIN) {
+    // one run of slots per coarse bucket: crank becomes the run's first slot
+    // in stg (exclusive scan by wave 0), cbase the run's global slot minus
+    // that local start (one cursor atomic per coarse bucket and block)
+    __syncthreads();
+    if (wave == 0) {
+      uint32_t run = 0;
+      for (int i0 = 0; i0 < p.ncb; i0 += 64) {
+        const int i = i0 + lane;
+        const uint32_t nc = i < p.ncb ? crank[i] : 0u;
+        uint32_t x = nc;  // inclusive scan over the wave
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const uint32_t y = __shfl_up(x, o);
+          if (lane >= o) x += y;
+        }
+        const uint32_t ex = run + x - nc;
+        if (i < p.ncb) {
+          cbase[i] = (nc ? atomicAdd(p.ccur + i, nc) : 0u) - ex;
+          crank[i] = ex;
+        }
+        run += __shfl(x, 63);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < LM_PER_LANE; j++) {
+      const uint32_t cb = (bcb[j >> 1] >> (16 * (j & 1))) & 0xffffu;
+      if (cb != 0xffffu) {
+        const uint32_t sl = atomicAdd(&crank[cb], 1u);
+        if (sl < LM_BLOCK) stg[sl] = bpay[j];  // the runs were counted for exactly these events
+      }
+    }
+  }
   __syncthreads();
   if constexpr (BIN) {
     // crank now holds the end of every run: wave k writes runs k, k + 4, ...,
