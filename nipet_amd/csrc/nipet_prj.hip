@@ -591,6 +591,23 @@ __global__ __launch_bounds__(fpg_threads<G>()) void k_fprj_g(GeomView g, const f
     two[r] = __any(iv > 0.0 && iv < (double)lmax);
   }
 
+  // z-columns of a chunk: each thread loads its NST values of the next chunk
+  // into registers before walking the current one, so the loads are in flight
+  // during the walk instead of between the two barriers
+  constexpr int NE = CH * ZSTRIDE * NB, NST = (NE + FPG_THREADS - 1) / FPG_THREADS;
+  float pre[NST];
+  auto fetch = [&](int c) {
+#pragma unroll
+    for (int i = 0; i < NST; i++) {
+      const int idx = tid + i * FPG_THREADS;
+      const int j = idx / (ZSTRIDE * NB), rem = idx - j * (ZSTRIDE * NB);
+      const int z = rem / NB, b = rem - z * NB, k = c + j;
+      float v = 0.f;
+      if (idx < NE && k < nseg && z < NZ) v = img[((int64_t)g.seg[beg + k].colz + z) * nbt + b0 + b];
+      pre[i] = v;
+    }
+  };
+  if (nseg > 0) fetch(0);
   for (int c = 0; c < nseg; c += CH) {
     const int nch = nseg - c < CH ? nseg - c : CH;
     // positions are relative to the chunk's middle segment: float positions
@@ -601,12 +618,14 @@ __global__ __launch_bounds__(fpg_threads<G>()) void k_fprj_g(GeomView g, const f
       const SegRec sg = g.seg[beg + (c + tid < nseg ? c + tid : c)];
       sr[tid] = c + tid < nseg ? make_float2((float)(sg.s - sb), sg.len) : make_float2(-1e30f, 0.f);
     }
-    for (int idx = tid; idx < CH * ZSTRIDE * NB; idx += FPG_THREADS) {
-      const int j = idx / (ZSTRIDE * NB), rem = idx - j * (ZSTRIDE * NB);
-      const int z = rem / NB, b = rem - z * NB, k = c + j;
-      float v = 0.f;
-      if (k < nseg && z < NZ) v = img[((int64_t)g.seg[beg + k].colz + z) * nbt + b0 + b];
-      zs[((z + FP_GUARD) * PS + j) * NB + b] = v;
+#pragma unroll
+    for (int i = 0; i < NST; i++) {
+      const int idx = tid + i * FPG_THREADS;
+      if (idx < NE) {
+        const int j = idx / (ZSTRIDE * NB), rem = idx - j * (ZSTRIDE * NB);
+        const int z = rem / NB, b = rem - z * NB;
+        zs[((z + FP_GUARD) * PS + j) * NB + b] = pre[i];
+      }
     }
 #pragma unroll
     for (int r = 0; r < R; r++) {
@@ -615,6 +634,7 @@ __global__ __launch_bounds__(fpg_threads<G>()) void k_fprj_g(GeomView g, const f
       L[r].tbr = iv > 0.0 ? (float)(((double)(ia + 1) - (double)z0[r][tid]) * iv - sb) : 3e38f;
     }
     __syncthreads();
+    if (c + CH < nseg) fetch(c + CH);
 #pragma unroll
     for (int r = 0; r < R; r++) {
       if (!live[r]) continue;
