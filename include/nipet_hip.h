@@ -155,11 +155,26 @@ int nipet_osem_corr(float *rho, float *an, const float *a, const float *n, const
  *   e = FP_sub(img) + rho[subs]; e = y[subs] / e (0 where |e| < 1e-6);
  *   b = BP_sub(e); img = mask ? img * (b / sens) : 0 (b/sens = 0 where |sens| < 1e-6).
  * prompts: uint16 [naw][nsn]; rho: float [naw][nsn]; sens: float image of this
- * subset; mask: uint8 image; work_sino: float [nprj][nsn]; work_img: float image
- * distinct from img and sens (aliased work buffers are rejected). */
+ * subset; mask: uint8 image; win: NULL (every segment of every ray) or the
+ * mask's segment windows from nipet_osem_window, valid when img is zero
+ * outside the mask (after any earlier update with that mask) - the projectors
+ * then skip the segments whose columns miss the mask; work_sino: float
+ * [nprj][nsn]; work_img: float image distinct from img and sens (aliased work
+ * buffers are rejected). */
 int nipet_osem_subset(const NipetGeom *g, float *img, const uint16_t *prompts, const float *rho,
-                      const float *sens, const uint8_t *mask, const int32_t *subs, int nprj,
-                      float *work_sino, float *work_img, void *stream);
+                      const float *sens, const uint8_t *mask, const int32_t *win,
+                      const int32_t *subs, int nprj, float *work_sino, float *work_img,
+                      void *stream);
+
+/* Segment windows of a reconstruction mask for the OSEM updates (no
+ * counterpart in the reference, which walks every segment): win[2 aw] and
+ * win[2 aw + 1] = first segment and segment count of bin aw's transaxial list
+ * whose columns hold a mask voxel (columns dilated by a box of half-width
+ * `dilate` voxels: 8, the 17-tap reach, with resolution modelling); 0, 0 when
+ * none. mask: device uint8 image; win: device int32 [naw][2]. Two 100 KB
+ * temporaries are allocated and freed stream-ordered on `stream`. */
+int nipet_osem_window(const NipetGeom *g, const uint8_t *mask, int dilate, int32_t *win,
+                      void *stream);
 
 /* nipet_osem_subset for a subset whose sensitivity image is not yet known
  * (the first OSEM iteration): sens = BP_sub(an[subs]) (the subset's image of
@@ -172,8 +187,8 @@ int nipet_osem_subset(const NipetGeom *g, float *img, const uint16_t *prompts, c
  * (nsn <= 837). */
 int nipet_osem_subset_sens(const NipetGeom *g, float *img, const uint16_t *prompts,
                            const float *rho, const float *an, float *sens, const uint8_t *mask,
-                           const int32_t *subs, int nprj, float *work_sino, float *work_sino2,
-                           float *work_img2, void *stream);
+                           const int32_t *win, const int32_t *subs, int nprj, float *work_sino,
+                           float *work_sino2, float *work_img2, void *stream);
 
 /* Batched OSEM subset update of nb images [voxel][nb]: prompts [naw][nsn][nb];
  * rho shared [naw][nsn] (rho_per_rep = 0) or [naw][nsn][nb]; sens shared
@@ -181,8 +196,8 @@ int nipet_osem_subset_sens(const NipetGeom *g, float *img, const uint16_t *promp
  * [nprj][nsn][nb]; work_img [voxel][nb]. */
 int nipet_osem_subset_b(const NipetGeom *g, float *img, const uint16_t *prompts,
                         const float *rho, int rho_per_rep, const float *sens, int sens_per_rep,
-                        const uint8_t *mask, const int32_t *subs, int nprj, int nb,
-                        float *work_sino, float *work_img, void *stream);
+                        const uint8_t *mask, const int32_t *win, const int32_t *subs, int nprj,
+                        int nb, float *work_sino, float *work_img, void *stream);
 
 /* One OSEM iteration over all subsets: the extension call petprj.osem
  * (osem_rec, prj/src/prj_module.cu:613-839; osem, prj/src/recon.cu:318-521)
@@ -199,8 +214,11 @@ int nipet_osem_subset_b(const NipetGeom *g, float *img, const uint16_t *prompts,
  *   prj/mmrrec.py:286-294);
  * sens: device float [nsub][320][320][127] (back projections of a n, not
  *   convolved); mask: device uint8 image; krnl: HOST float [3][17] or NULL.
- * Temporaries (rho, a n, one work sinogram and image, two more images with
- * resolution modelling) are allocated and freed stream-ordered on `stream`. */
+ * Subsets after the first walk only the mask's segment windows
+ * (nipet_osem_window): img is then zero outside the mask.
+ * Temporaries (rho, a n, one work sinogram and image, the windows, two more
+ * images with resolution modelling) are allocated and freed stream-ordered on
+ * `stream`. */
 int nipet_osem_iter(const NipetGeom *g, float *img, const uint16_t *prompts, const float *r,
                     const float *s, const float *n, const float *a, const int32_t *subs,
                     int nsub, int stride, const float *sens, const uint8_t *mask,
@@ -220,9 +238,9 @@ int nipet_psf_conv(float *out, const float *in, float *tmp, const float *krnl, i
  * images [voxel][nb]. */
 int nipet_osem_subset_rm_b(const NipetGeom *g, float *img, const uint16_t *prompts,
                            const float *rho, int rho_per_rep, const float *sens,
-                           int sens_per_rep, const uint8_t *mask, const int32_t *subs, int nprj,
-                           int nb, const float *krnl, float *work_sino, float *work_img,
-                           float *work_img2, void *stream);
+                           int sens_per_rep, const uint8_t *mask, const int32_t *win,
+                           const int32_t *subs, int nprj, int nb, const float *krnl,
+                           float *work_sino, float *work_img, float *work_img2, void *stream);
 
 /* ---- list-mode histogramming (mmr_lmproc.hist: lm/src/lm_module.cu:174-391,
  * lmproc lm/src/lmproc.cu:12-230, gpu_hst / hst lm/src/hst.cu:37-528) ----

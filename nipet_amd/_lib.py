@@ -84,16 +84,17 @@ def lib():
             'nipet_remgaps_u16b': ([P, P, P, I, I, I, I, I, I, I, I, P], I),
             'nipet_tune_set': ([I, I], I),
             'nipet_psf_conv': ([P, P, P, P, I, P], I),
-            'nipet_osem_subset_rm_b': ([P, P, P, P, I, P, I, P, P, I, I, P, P, P, P, P], I),
+            'nipet_osem_subset_rm_b': ([P, P, P, P, I, P, I, P, P, P, I, I, P, P, P, P, P], I),
             'nipet_tune_get': ([I, P], I),
             'nipet_tune_set_run': ([I, I], I),
             'nipet_tune_get_run': ([I, P, P], I),
-            'nipet_osem_subset': ([P, P, P, P, P, P, P, I, P, P, P], I),
-            'nipet_osem_subset_sens': ([P, P, P, P, P, P, P, P, I, P, P, P, P], I),
+            'nipet_osem_subset': ([P, P, P, P, P, P, P, P, I, P, P, P], I),
+            'nipet_osem_window': ([P, P, I, P, P], I),
+            'nipet_osem_subset_sens': ([P, P, P, P, P, P, P, P, P, I, P, P, P, P], I),
             'nipet_osem_iter': ([P, P, P, P, P, P, P, P, I, I, P, P, P, P], I),
             'nipet_fprj_b': ([P, P, P, P, I, I, I, P], I),
             'nipet_bprj_b': ([P, P, P, P, I, I, P], I),
-            'nipet_osem_subset_b': ([P, P, P, P, I, P, I, P, P, I, I, P, P, P], I),
+            'nipet_osem_subset_b': ([P, P, P, P, I, P, I, P, P, P, I, I, P, P, P], I),
             'nipet_lm_create': ([ct.POINTER(NipetLmConfig), ct.POINTER(P)], I),
             'nipet_lm_chunk': ([P, P, I64, ct.POINTER(NipetLmOut), P], I),
             'nipet_lm_status': ([P, P, P, P], I),

@@ -80,7 +80,23 @@ static GeomView view(const Geom *g) {
 // images are [voxel][nbt], sinograms [row][nsn][nbt] (replicate innermost).
 struct Batch {
   int nbt, b0;
+  // optional per-bin segment window [first, first + count) of the bin's list
+  // (nipet_osem_window): the segments whose columns meet the reconstruction
+  // mask; the rest carry zero image values (FP) or feed voxels the masked
+  // update discards (BP)
+  const int32_t *win = nullptr;
 };
+
+// The bin's segment list, or its window.
+__device__ inline void seg_range(const GeomView &g, const Batch &bt, int aw, int &beg,
+                                 int &nseg) {
+  beg = g.off[aw];
+  nseg = g.off[aw + 1] - beg;
+  if (bt.win) {
+    beg += bt.win[2 * aw];
+    nseg = bt.win[2 * aw + 1];
+  }
+}
 
 template <int NB>
 __device__ inline void ldv(float (&r)[NB], const float *p) {
@@ -282,7 +298,8 @@ __global__ __launch_bounds__(FP_THREADS, NB >= 4 ? 4 : 6) void k_fprj(GeomView g
     for (int i = tid; i < nsn * NB; i += FP_THREADS) out[(i / NB) * nbt + i % NB] = 0.f;
     return;
   }
-  const int beg = g.off[aw], nseg = g.off[aw + 1] - beg;
+  int beg, nseg;
+  seg_range(g, bt, aw, beg, nseg);
   const double atn = g.atn[aw];
   const float lmax = g.lmax[aw];
   float *zs = lds;
@@ -548,7 +565,8 @@ __global__ __launch_bounds__(fpg_threads<G>()) void k_fprj_g(GeomView g, const f
     for (int i = tid; i < nsn * NB; i += FPG_THREADS) out[(i / NB) * nbt + i % NB] = 0.f;
     return;
   }
-  const int beg = g.off[aw], nseg = g.off[aw + 1] - beg;
+  int beg, nseg;
+  seg_range(g, bt, aw, beg, nseg);
   const double atn = g.atn[aw];
   const float lmax = g.lmax[aw];
   float *zs = lds;
@@ -691,6 +709,54 @@ __global__ __launch_bounds__(fpg_threads<G>()) void k_fprj_g(GeomView g, const f
   });
 }
 
+// Segment windows of the reconstruction mask (nipet_osem_window): m2[c] = 1
+// when column c = u + NX v of the mask holds any voxel, optionally dilated by
+// a box of half-width r (the resolution model's 17-tap reach); then, one wave
+// per bin, the first and last segment of the bin's list whose column is set.
+__global__ void k_mask_cols(const uint8_t *__restrict__ mask, uint8_t *__restrict__ m2) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= NX * NY) return;
+  const uint8_t *p = mask + (int64_t)c * NZ;
+  int any = 0;
+  for (int z = 0; z < NZ; z++) any |= p[z];
+  m2[c] = any != 0;
+}
+
+__global__ void k_mask_dilate(const uint8_t *__restrict__ m2, uint8_t *__restrict__ m2d, int r) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= NX * NY) return;
+  const int u = c % NX, v = c / NX;
+  int any = 0;
+  for (int vv = v - r; vv <= v + r; vv++) {
+    if (vv < 0 || vv >= NY) continue;
+    for (int uu = u - r; uu <= u + r; uu++)
+      if (uu >= 0 && uu < NX) any |= m2[uu + NX * vv];
+  }
+  m2d[c] = any != 0;
+}
+
+__global__ __launch_bounds__(256) void k_seg_window(GeomView g, const uint8_t *__restrict__ m2,
+                                                    int32_t *__restrict__ win) {
+  const int lane = threadIdx.x & 63;
+  const int aw = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (aw >= g.naw) return;  // wave-uniform
+  const int beg = g.off[aw], n = g.off[aw + 1] - beg;
+  int first = -1, last = -1;
+  for (int k0 = 0; k0 < n; k0 += 64) {
+    const int k = k0 + lane;
+    const bool in = k < n && m2[g.seg[beg + k].colz / NZ] != 0;
+    const unsigned long long m = __ballot(in);
+    if (m) {
+      if (first < 0) first = k0 + __ffsll((long long)m) - 1;
+      last = k0 + 63 - __clzll(m);
+    }
+  }
+  if (lane == 0) {
+    win[2 * aw] = first < 0 ? 0 : first;
+    win[2 * aw + 1] = first < 0 ? 0 : last - first + 1;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Back projection (adjoint of the emission FP) as a gather, accumulated into img.
 //
@@ -747,7 +813,8 @@ __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g
   // sinogram rows in subset order, or (by_bin) the full [naw][nsn] sinogram
   // read at the bin's own row
   const float *in = sino + (size_t)(by_bin ? aw : row) * nsn * nbt + b0;
-  const int beg = g.off[aw], nseg = g.off[aw + 1] - beg;
+  int beg, nseg;
+  seg_range(g, bt, aw, beg, nseg);
   const double atn = g.atn[aw];
 
   // LDS: sinogram row, zero row at index nsn, and the LOR -> sinogram tables
@@ -1101,6 +1168,7 @@ static std::atomic<int> g_run_bp[3] = {{bp_spw_a<1>()}, {bp_spw_a<2>()}, {bp_spw
 template <int NB>
 static void launch_bprj_w(const Geom *g, float *img, const float *sino, const int32_t *subs,
                           int nprj, Batch bt, int by_bin, hipStream_t st) {
+  // bt.win: the bin's segment window, or every segment when NULL
   const int run = g_run_bp[NB == 1 ? 0 : (NB == 2 ? 1 : 2)].load();
   if (run == bp_spw_b<NB>())
     k_bprj<NB, bp_spw_b<NB>()><<<nprj, BP_THREADS, 0, st>>>(view(g), img, sino, subs, bt, by_bin);
@@ -1111,11 +1179,11 @@ static void launch_bprj_w(const Geom *g, float *img, const float *sino, const in
 template <int MODE>
 static void launch_fprj(const Geom *g, float *sino, const float *img, const int32_t *subs,
                         int nprj, const uint16_t *prompts, const float *rho, int rho_nb, int nb,
-                        hipStream_t st) {
+                        hipStream_t st, const int32_t *win = nullptr) {
   const int wmax = g_width_fp.load();
   for (int b0 = 0; b0 < nb;) {
     const int rem = nb - b0;
-    const Batch bt{nb, b0};
+    const Batch bt{nb, b0, win};
     if (rem >= 4 && wmax >= 4) {
       k_fprj<MODE, 4><<<nprj, FP_THREADS, 0, st>>>(view(g), img, sino, subs, prompts, rho,
                                                    rho_nb, bt);
@@ -1133,11 +1201,12 @@ static void launch_fprj(const Geom *g, float *sino, const float *img, const int3
 }
 
 static void launch_bprj(const Geom *g, float *img, const float *sino, const int32_t *subs,
-                        int nprj, int nb, hipStream_t st, int by_bin = 0) {
+                        int nprj, int nb, hipStream_t st, int by_bin = 0,
+                        const int32_t *win = nullptr) {
   const int wmax = g_width_bp.load();
   for (int b0 = 0; b0 < nb;) {
     const int rem = nb - b0;
-    const Batch bt{nb, b0};
+    const Batch bt{nb, b0, win};
     if (rem >= 4 && wmax >= 4) {
       launch_bprj_w<4>(g, img, sino, subs, nprj, bt, by_bin, st);
       b0 += 4;
@@ -1248,6 +1317,31 @@ int nipet_osem_sens(const NipetGeom *g, float *sens, const float *an, const int3
   prof_begin(PK_BPRJ, st);
   launch_bprj(g, sens, an, subs, nprj, 1, st, 1);
   prof_end(PK_BPRJ, st);
+  NIPET_HIP(hipGetLastError());
+  return NIPET_OK;
+}
+
+int nipet_osem_window(const NipetGeom *g, const uint8_t *mask, int dilate, int32_t *win,
+                      void *stream) {
+  int rc = check_geom(g, "nipet_osem_window");
+  if (rc) return rc;
+  if (!mask || !win || dilate < 0 || dilate > 64) {
+    set_error("nipet_osem_window: NULL buffer or dilate outside 0 .. 64");
+    return NIPET_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  uint8_t *m2 = nullptr, *m2d = nullptr;
+  hipError_t e = hipMallocAsync((void **)&m2, (size_t)NX * NY, st);
+  if (e == hipSuccess && dilate > 0) e = hipMallocAsync((void **)&m2d, (size_t)NX * NY, st);
+  if (e != hipSuccess) {
+    if (m2) (void)hipFreeAsync(m2, st);
+    return hip_fail(e, "nipet_osem_window: hipMallocAsync");
+  }
+  k_mask_cols<<<(NX * NY + 255) / 256, 256, 0, st>>>(mask, m2);
+  if (dilate > 0) k_mask_dilate<<<(NX * NY + 255) / 256, 256, 0, st>>>(m2, m2d, dilate);
+  k_seg_window<<<(g->naw + 3) / 4, 256, 0, st>>>(view(g), dilate > 0 ? m2d : m2, win);
+  (void)hipFreeAsync(m2, st);
+  if (m2d) (void)hipFreeAsync(m2d, st);
   NIPET_HIP(hipGetLastError());
   return NIPET_OK;
 }
@@ -1433,8 +1527,8 @@ int nipet_osem_corr(float *rho, float *an, const float *a, const float *n, const
 
 int nipet_osem_subset_b(const NipetGeom *g, float *img, const uint16_t *prompts,
                         const float *rho, int rho_per_rep, const float *sens, int sens_per_rep,
-                        const uint8_t *mask, const int32_t *subs, int nprj, int nb,
-                        float *work_sino, float *work_img, void *stream) {
+                        const uint8_t *mask, const int32_t *win, const int32_t *subs, int nprj,
+                        int nb, float *work_sino, float *work_img, void *stream) {
   int rc = check_geom(g, "nipet_osem_subset");
   if (rc) return rc;
   if (!subs || nprj <= 0 || nprj > g->naw) {
@@ -1449,13 +1543,13 @@ int nipet_osem_subset_b(const NipetGeom *g, float *img, const uint16_t *prompts,
   }
   hipStream_t st = (hipStream_t)stream;
   prof_begin(PK_FPRJ, st);
-  launch_fprj<2>(g, work_sino, img, subs, nprj, prompts, rho, rho_per_rep ? 1 : 0, nb, st);
+  launch_fprj<2>(g, work_sino, img, subs, nprj, prompts, rho, rho_per_rep ? 1 : 0, nb, st, win);
   prof_end(PK_FPRJ, st);
   prof_begin(PK_FILL, st);
   k_fill<<<ew_blocks(NVOX * nb), 256, 0, st>>>(work_img, 0.f, NVOX * nb);
   prof_end(PK_FILL, st);
   prof_begin(PK_BPRJ, st);
-  launch_bprj(g, work_img, work_sino, subs, nprj, nb, st);
+  launch_bprj(g, work_img, work_sino, subs, nprj, nb, st, 0, win);
   prof_end(PK_BPRJ, st);
   prof_begin(PK_OSEM_UPDATE, st);
   k_osem_update<<<ew_blocks(NVOX * nb), 256, 0, st>>>(img, work_img, sens, mask, NVOX, nb,
@@ -1467,9 +1561,9 @@ int nipet_osem_subset_b(const NipetGeom *g, float *img, const uint16_t *prompts,
 
 int nipet_osem_subset_rm_b(const NipetGeom *g, float *img, const uint16_t *prompts,
                            const float *rho, int rho_per_rep, const float *sens,
-                           int sens_per_rep, const uint8_t *mask, const int32_t *subs, int nprj,
-                           int nb, const float *krnl, float *work_sino, float *work_img,
-                           float *work_img2, void *stream) {
+                           int sens_per_rep, const uint8_t *mask, const int32_t *win,
+                           const int32_t *subs, int nprj, int nb, const float *krnl,
+                           float *work_sino, float *work_img, float *work_img2, void *stream) {
   int rc = check_geom(g, "nipet_osem_subset_rm");
   if (rc) return rc;
   if (!subs || nprj <= 0 || nprj > g->naw) {
@@ -1487,13 +1581,14 @@ int nipet_osem_subset_rm_b(const NipetGeom *g, float *img, const uint16_t *promp
   // resolution-modelled image H x (work_img is the convolution scratch)
   if ((rc = psf_conv(work_img2, img, work_img, krnl, nb, st))) return rc;
   prof_begin(PK_FPRJ, st);
-  launch_fprj<2>(g, work_sino, work_img2, subs, nprj, prompts, rho, rho_per_rep ? 1 : 0, nb, st);
+  launch_fprj<2>(g, work_sino, work_img2, subs, nprj, prompts, rho, rho_per_rep ? 1 : 0, nb, st,
+                 win);
   prof_end(PK_FPRJ, st);
   prof_begin(PK_FILL, st);
   k_fill<<<ew_blocks(NVOX * nb), 256, 0, st>>>(work_img, 0.f, NVOX * nb);
   prof_end(PK_FILL, st);
   prof_begin(PK_BPRJ, st);
-  launch_bprj(g, work_img, work_sino, subs, nprj, nb, st);
+  launch_bprj(g, work_img, work_sino, subs, nprj, nb, st, 0, win);
   prof_end(PK_BPRJ, st);
   // H^T of the back projection (symmetric kernels: H), into work_img2
   if ((rc = psf_conv(work_img2, work_img, work_img, krnl, nb, st))) return rc;
@@ -1506,16 +1601,17 @@ int nipet_osem_subset_rm_b(const NipetGeom *g, float *img, const uint16_t *promp
 }
 
 int nipet_osem_subset(const NipetGeom *g, float *img, const uint16_t *prompts, const float *rho,
-                      const float *sens, const uint8_t *mask, const int32_t *subs, int nprj,
-                      float *work_sino, float *work_img, void *stream) {
-  return nipet_osem_subset_b(g, img, prompts, rho, 0, sens, 0, mask, subs, nprj, 1, work_sino,
-                             work_img, stream);
+                      const float *sens, const uint8_t *mask, const int32_t *win,
+                      const int32_t *subs, int nprj, float *work_sino, float *work_img,
+                      void *stream) {
+  return nipet_osem_subset_b(g, img, prompts, rho, 0, sens, 0, mask, win, subs, nprj, 1,
+                             work_sino, work_img, stream);
 }
 
 int nipet_osem_subset_sens(const NipetGeom *g, float *img, const uint16_t *prompts,
                            const float *rho, const float *an, float *sens, const uint8_t *mask,
-                           const int32_t *subs, int nprj, float *work_sino, float *work_sino2,
-                           float *work_img2, void *stream) {
+                           const int32_t *win, const int32_t *subs, int nprj, float *work_sino,
+                           float *work_sino2, float *work_img2, void *stream) {
   int rc = check_geom(g, "nipet_osem_subset_sens");
   if (rc) return rc;
   if (!subs || nprj <= 0 || nprj > g->naw) {
@@ -1532,7 +1628,7 @@ int nipet_osem_subset_sens(const NipetGeom *g, float *img, const uint16_t *promp
   hipStream_t st = (hipStream_t)stream;
   const int64_t nel = (int64_t)nprj * g->nsn;
   prof_begin(PK_FPRJ, st);
-  launch_fprj<2>(g, work_sino, img, subs, nprj, prompts, rho, 0, 1, st);
+  launch_fprj<2>(g, work_sino, img, subs, nprj, prompts, rho, 0, 1, st, win);
   prof_end(PK_FPRJ, st);
   prof_begin(PK_OTHER, st);
   k_pack_pair<<<ew_blocks(nel), 256, 0, st>>>((float2 *)work_sino2, work_sino, an, subs, g->nsn,
@@ -1544,7 +1640,7 @@ int nipet_osem_subset_sens(const NipetGeom *g, float *img, const uint16_t *promp
   // both images through one traversal of the rays (independent of the
   // batch autotuner's width)
   prof_begin(PK_BPRJ_PAIR, st);
-  launch_bprj_w<2>(g, work_img2, work_sino2, subs, nprj, Batch{2, 0}, 0, st);
+  launch_bprj_w<2>(g, work_img2, work_sino2, subs, nprj, Batch{2, 0, win}, 0, st);
   prof_end(PK_BPRJ_PAIR, st);
   prof_begin(PK_OSEM_UPDATE, st);
   k_osem_update_pair<<<ew_blocks(NVOX), 256, 0, st>>>(img, sens, (const float2 *)work_img2, mask,
@@ -1581,9 +1677,9 @@ int nipet_osem_iter(const NipetGeom *g, float *img, const uint16_t *prompts, con
   // its device copies in every call, prj/src/recon.cu:333-419, 511-520)
   float *rho = nullptr, *an = nullptr, *wsino = nullptr, *wimg = nullptr, *wimg2 = nullptr,
         *srm = nullptr;
-  int32_t *dsubs = nullptr;
+  int32_t *dsubs = nullptr, *win = nullptr;
   auto release = [&]() {
-    void *ps[] = {rho, an, wsino, wimg, wimg2, srm, dsubs};
+    void *ps[] = {rho, an, wsino, wimg, wimg2, srm, dsubs, win};
     for (void *q : ps)
       if (q) (void)hipFreeAsync(q, st);
   };
@@ -1599,7 +1695,8 @@ int nipet_osem_iter(const NipetGeom *g, float *img, const uint16_t *prompts, con
   if ((rc = get((void **)&rho, nel * sizeof(float))) || (rc = get((void **)&an, nel * sizeof(float))) ||
       (rc = get((void **)&wsino, (size_t)maxc * g->nsn * sizeof(float))) ||
       (rc = get((void **)&wimg, NVOX * sizeof(float))) ||
-      (rc = get((void **)&dsubs, (size_t)nsub * stride * sizeof(int32_t))))
+      (rc = get((void **)&dsubs, (size_t)nsub * stride * sizeof(int32_t))) ||
+      (rc = get((void **)&win, (size_t)g->naw * 2 * sizeof(int32_t))))
     return rc;
   if (rm && ((rc = get((void **)&wimg2, NVOX * sizeof(float))) ||
              (rc = get((void **)&srm, NVOX * sizeof(float)))))
@@ -1612,18 +1709,26 @@ int nipet_osem_iter(const NipetGeom *g, float *img, const uint16_t *prompts, con
   }
   // rho = (r + s) / (a n) once per call (prj/src/recon.cu:381-398)
   k_osem_corr<<<ew_blocks(nel), 256, 0, st>>>(rho, an, a, n, r, s, nel);
+  // the mask's segment windows (dilated by the resolution model's reach):
+  // after the first update img is zero outside the mask, so the later subsets
+  // walk only the segments whose columns meet it
+  if ((rc = nipet_osem_window(g, mask, rm ? PSF_R : 0, win, stream))) {
+    release();
+    return rc;
+  }
   for (int i = 0; i < nsub && rc == NIPET_OK; i++) {
     const int c = subs[(int64_t)i * stride];
     const int32_t *rows = dsubs + (int64_t)i * stride + 1;
     const float *si = sens + (int64_t)i * NVOX;
+    const int32_t *wi = i > 0 ? win : nullptr;
     if (rm) {
       // the reference convolves the sensitivity images in every call
       // (prj/src/recon.cu:441-445); here one at a time, before its subset
       if ((rc = psf_conv(srm, si, wimg, krnl, 1, st))) break;
-      rc = nipet_osem_subset_rm_b(g, img, prompts, rho, 0, srm, 0, mask, rows, c, 1, krnl, wsino,
-                                  wimg, wimg2, stream);
+      rc = nipet_osem_subset_rm_b(g, img, prompts, rho, 0, srm, 0, mask, wi, rows, c, 1, krnl,
+                                  wsino, wimg, wimg2, stream);
     } else {
-      rc = nipet_osem_subset_b(g, img, prompts, rho, 0, si, 0, mask, rows, c, 1, wsino, wimg,
+      rc = nipet_osem_subset_b(g, img, prompts, rho, 0, si, 0, mask, wi, rows, c, 1, wsino, wimg,
                                stream);
     }
   }

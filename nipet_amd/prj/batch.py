@@ -146,7 +146,7 @@ def prepare_groups(scanner_params, prompt_groups, asng, nsng, rsng=None, ssng=No
         work = None if first is None else (first.work_sino, first.work_img, first.work_img2)
         if first is not None and not per:
             st = OsemBatch(first.geom, first.subsets, p, first.rho, False, first.sens,
-                           first.mask, g, work=work, krnl=first.krnl)
+                           first.mask, g, work=work, krnl=first.krnl, win=first.win)
         else:
             def sl(x):
                 return x[..., off:off + g].contiguous() if _per_rep(x) else x
@@ -181,15 +181,15 @@ class OsemGraph:
             first = st.subsets[0][:1].contiguous()
             _lib.check(_lib.lib().nipet_osem_subset_b(
                 st.geom.handle, _lib.ptr(img), _lib.ptr(st.prompts), _lib.ptr(st.rho),
-                int(st.rho_per_rep), _lib.ptr(st.sens[0]), 0, _lib.ptr(st.mask),
+                int(st.rho_per_rep), _lib.ptr(st.sens[0]), 0, _lib.ptr(st.mask), None,
                 _lib.ptr(first), 1, st.nb, _lib.ptr(st.work_sino), _lib.ptr(st.work_img),
                 _lib.stream_handle()), 'nipet_osem_subset_b')
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
-            for _ in range(self.niter):
+            for it in range(self.niter):
                 for st, img in zip(states, self.imgs):
-                    osem_iteration_batch(st, img)
+                    osem_iteration_batch(st, img, first_full=it == 0)
         torch.cuda.synchronize()
 
     def run_groups(self, img0=None):
@@ -318,9 +318,9 @@ def bootstrap_osem(words, scanner_params, asng, nsng, rsng=None, ssng=None, nrep
     else:
         imgs = [torch.ones((320, 320, 127, x.nb), dtype=torch.float32, device='cuda')
                 for x in states]
-        for _ in range(itr):
+        for it in range(itr):
             for x, im in zip(states, imgs):
-                osem_iteration_batch(x, im)
+                osem_iteration_batch(x, im, first_full=it == 0)
         img = _join(imgs)
     mark('osem_s')
     info['sens'] = st.sens
@@ -345,9 +345,9 @@ def frames_osem(words, scanner_params, frames, asng, nsng, rho_frames, itr=4, tu
     else:
         imgs = [torch.ones((320, 320, 127, x.nb), dtype=torch.float32, device='cuda')
                 for x in states]
-        for _ in range(itr):
+        for it in range(itr):
             for x, im in zip(states, imgs):
-                osem_iteration_batch(x, im)
+                osem_iteration_batch(x, im, first_full=it == 0)
         img = _join(imgs)
     return img, info
 

@@ -23,7 +23,7 @@ from ..geom import get_geom
 
 log = logging.getLogger(__name__)
 
-__all__ = ['OsemState', 'osem_prepare', 'osem_run', 'sensitivity', 'cylinder_mask',
+__all__ = ['OsemState', 'osem_prepare', 'osem_run', 'sensitivity', 'cylinder_mask', 'mask_window',
            'prompts_u16']
 
 
@@ -73,6 +73,19 @@ def cylinder_mask(Cnt, radius=29.):
     return m
 
 
+def mask_window(geom, mask, psf=False):
+    """Device int32 (Naw, 2) segment windows of a uint8 device mask
+    (``nipet_osem_window``): the first segment and the count of each bin's
+    transaxial list whose columns hold a mask voxel; with resolution modelling
+    the columns are dilated by the kernel's reach (8 voxels)."""
+    torch = _torch()
+    win = torch.empty((geom.naw, 2), dtype=torch.int32, device='cuda')
+    _lib.check(_lib.lib().nipet_osem_window(geom.handle, _lib.ptr(mask), 8 if psf else 0,
+                                            _lib.ptr(win), _lib.stream_handle()),
+               'nipet_osem_window')
+    return win
+
+
 def prompts_u16(psng):
     """Prompt counts as uint16, saturated to 0 .. 65535 (fractional values
     truncated), for NumPy arrays and torch tensors alike (the result stays in
@@ -104,10 +117,14 @@ def _dev(x, dtype):
 class OsemState:
     """Everything one OSEM reconstruction keeps on the device."""
 
-    def __init__(self, geom, subsets, prompts, rho, an, sens, mask, krnl=None, pending=None):
+    def __init__(self, geom, subsets, prompts, rho, an, sens, mask, krnl=None, pending=None,
+                 win=None):
         torch = _torch()
         self.geom = geom
         self.subsets = subsets
+        # the mask's segment windows (nipet_osem_window), used once the image
+        # is zero outside the mask
+        self.win = win
         self.nprj = max(len(s) for s in subsets)
         self.prompts, self.rho, self.an, self._sens, self.mask = prompts, rho, an, sens, mask
         # subsets whose sensitivity image is still to be back-projected (in
@@ -123,7 +140,9 @@ class OsemState:
     @property
     def sens(self):
         """Sensitivity images [nsub, 320, 320, 127], completing any not yet
-        back-projected."""
+        back-projected. (An image back-projected inside an update walked the
+        mask's segment windows: it is exact on the mask, which is all the
+        update reads.)"""
         for k in sorted(self.pending):
             sensitivity(self.geom, self.an, self.subsets[k], out=self._sens[k])
         self.pending.clear()
@@ -231,18 +250,23 @@ def osem_prepare(scanner_params, psng, asng, nsng, rsng=None, ssng=None, mask_ra
         mask = cylinder_mask(Cnt, mask_radius)
     msk = _dev(mask if isinstance(mask, torch.Tensor) else np.asarray(mask, np.uint8),
                torch.uint8)
-    return OsemState(geom, subs, prompts, rho, an, sens, msk, krnl, pending)
+    return OsemState(geom, subs, prompts, rho, an, sens, msk, krnl, pending,
+                     mask_window(geom, msk, krnl is not None))
 
 
-def osem_iteration(state, img):
-    """One OSEM iteration (all subsets) in place on the device image ``img``."""
+def osem_iteration(state, img, first_full=True):
+    """One OSEM iteration (all subsets) in place on the device image ``img``.
+    Every update after the first walks the mask's segment windows (the image
+    is then zero outside the mask); the first does too when ``first_full`` is
+    False (an image already masked by an earlier update)."""
     st = _lib.stream_handle()
     L = _lib.lib()
     for k, subs in enumerate(state.subsets):
+        win = None if (k == 0 and first_full) else _lib.ptr(state.win)
         if state.krnl is not None:
             _lib.check(L.nipet_osem_subset_rm_b(
                 state.geom.handle, _lib.ptr(img), _lib.ptr(state.prompts), _lib.ptr(state.rho), 0,
-                _lib.ptr(state.sens[k]), 0, _lib.ptr(state.mask), _lib.ptr(subs), len(subs), 1,
+                _lib.ptr(state.sens[k]), 0, _lib.ptr(state.mask), win, _lib.ptr(subs), len(subs), 1,
                 state.krnl.ctypes.data, _lib.ptr(state.work_sino), _lib.ptr(state.work_img),
                 _lib.ptr(state.work_img2), st), 'nipet_osem_subset_rm_b')
             continue
@@ -250,7 +274,7 @@ def osem_iteration(state, img):
             ws2, wi2 = state.pair_work()
             _lib.check(L.nipet_osem_subset_sens(
                 state.geom.handle, _lib.ptr(img), _lib.ptr(state.prompts), _lib.ptr(state.rho),
-                _lib.ptr(state.an), _lib.ptr(state._sens[k]), _lib.ptr(state.mask),
+                _lib.ptr(state.an), _lib.ptr(state._sens[k]), _lib.ptr(state.mask), win,
                 _lib.ptr(subs), len(subs), _lib.ptr(state.work_sino), _lib.ptr(ws2),
                 _lib.ptr(wi2), st), 'nipet_osem_subset_sens')
             state.pending.discard(k)
@@ -259,7 +283,7 @@ def osem_iteration(state, img):
             continue
         _lib.check(L.nipet_osem_subset(state.geom.handle, _lib.ptr(img), _lib.ptr(state.prompts),
                                        _lib.ptr(state.rho), _lib.ptr(state._sens[k]),
-                                       _lib.ptr(state.mask), _lib.ptr(subs), len(subs),
+                                       _lib.ptr(state.mask), win, _lib.ptr(subs), len(subs),
                                        _lib.ptr(state.work_sino), _lib.ptr(state.work_img), st),
                    'nipet_osem_subset')
 
@@ -272,7 +296,7 @@ def osem_run(state, niter, img0=None, check_counts=True):
         else _dev(img0, torch.float32).clone()
     state.iterations = 0
     for k in range(niter):
-        osem_iteration(state, img)
+        osem_iteration(state, img, first_full=k == 0)
         state.iterations = k + 1
         if check_counts and float(torch.nansum(img)) < 0.1:
             log.warning('it seems there is not enough true data to render reasonable image')
@@ -288,10 +312,11 @@ class OsemBatch:
     [320, 320, 127, nb] (replicate innermost)."""
 
     def __init__(self, geom, subsets, prompts, rho, rho_per_rep, sens, mask, nb, work=None,
-                 krnl=None):
+                 krnl=None, win=None):
         torch = _torch()
         self.geom, self.subsets, self.nb = geom, subsets, nb
         self.krnl = krnl
+        self.win = mask_window(geom, mask, krnl is not None) if win is None else win
         self.prompts, self.rho, self.rho_per_rep, self.sens, self.mask = (
             prompts, rho, rho_per_rep, sens, mask)
         self.nprj = max(len(s) for s in subsets)
@@ -373,21 +398,23 @@ def osem_prepare_batch(scanner_params, prompts, asng, nsng, rsng=None, ssng=None
     return OsemBatch(geom, subs, p, rho, per_rep, sens, msk, nb, work=work, krnl=krnl)
 
 
-def osem_iteration_batch(state, img):
+def osem_iteration_batch(state, img, first_full=True):
+    """One OSEM iteration of all nb images; windows as ``osem_iteration``."""
     st = _lib.stream_handle()
     L = _lib.lib()
     for k, subs in enumerate(state.subsets):
+        win = None if (k == 0 and first_full) else _lib.ptr(state.win)
         if state.krnl is not None:
             _lib.check(L.nipet_osem_subset_rm_b(
                 state.geom.handle, _lib.ptr(img), _lib.ptr(state.prompts), _lib.ptr(state.rho),
-                int(state.rho_per_rep), _lib.ptr(state.sens[k]), 0, _lib.ptr(state.mask),
+                int(state.rho_per_rep), _lib.ptr(state.sens[k]), 0, _lib.ptr(state.mask), win,
                 _lib.ptr(subs), len(subs), state.nb, state.krnl.ctypes.data,
                 _lib.ptr(state.work_sino), _lib.ptr(state.work_img), _lib.ptr(state.work_img2),
                 st), 'nipet_osem_subset_rm_b')
             continue
         _lib.check(L.nipet_osem_subset_b(
             state.geom.handle, _lib.ptr(img), _lib.ptr(state.prompts), _lib.ptr(state.rho),
-            int(state.rho_per_rep), _lib.ptr(state.sens[k]), 0, _lib.ptr(state.mask),
+            int(state.rho_per_rep), _lib.ptr(state.sens[k]), 0, _lib.ptr(state.mask), win,
             _lib.ptr(subs), len(subs), state.nb, _lib.ptr(state.work_sino),
             _lib.ptr(state.work_img), st), 'nipet_osem_subset_b')
 
@@ -398,6 +425,6 @@ def osem_run_batch(state, niter, img0=None):
     torch = _torch()
     img = torch.ones((320, 320, 127, state.nb), dtype=torch.float32, device='cuda') \
         if img0 is None else _dev(img0, torch.float32).clone()
-    for _ in range(niter):
-        osem_iteration_batch(state, img)
+    for k in range(niter):
+        osem_iteration_batch(state, img, first_full=k == 0)
     return img

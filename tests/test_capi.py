@@ -44,9 +44,11 @@ def test_status_codes_without_gpu(L):
     assert L.nipet_geom_create(None, ct.byref(h)) == -1
     assert L.nipet_osem_corr(None, None, None, None, None, None, 10, None) == -1
     assert b'nipet_osem_corr' in L.nipet_last_error()
-    assert L.nipet_osem_subset_sens(None, None, None, None, None, None, None, None, 10, None,
-                                    None, None, None) == -1
+    assert L.nipet_osem_subset_sens(None, None, None, None, None, None, None, None, None, 10,
+                                    None, None, None, None) == -1
     assert b'nipet_osem_subset_sens: NULL geometry' in L.nipet_last_error()
+    assert L.nipet_osem_window(None, None, 0, None, None) == -1
+    assert b'nipet_osem_window: NULL geometry' in L.nipet_last_error()
     assert L.nipet_osem_iter(None, None, None, None, None, None, None, None, 1, 2, None, None,
                              None, None) == -1
     assert b'nipet_osem_iter: NULL geometry' in L.nipet_last_error()

@@ -71,8 +71,8 @@ def test_graph_matches_eager_and_replays(torch, params, data):
     prompts, _ = bootstrap_prompts(data['words'], params, 3, seed=5)
     st = osem_prepare_batch(params, prompts, data['attn'], data['norm'], data['rs'])
     eager = torch.ones((320, 320, 127, 3), device='cuda')
-    for _ in range(ITR):
-        osem_iteration_batch(st, eager)
+    for it in range(ITR):
+        osem_iteration_batch(st, eager, first_full=it == 0)
     g = OsemGraph(st, ITR)
     a = g.run().clone()
     torch.cuda.synchronize()

@@ -55,15 +55,26 @@ def test_osem_corr_guards(torch, params):
     assert rho.tolist() == [1.5, 0.0, 0.0, -2.0]
 
 
-def test_osem_mini_vs_oracle(torch, params, subsets):
+def _mask(params, kind):
+    """'planes': every column (no segment window skips anything); 'disc': the
+    reference's cylinder of radius 20 cm (the windows skip the segments outside)."""
+    if kind == 'disc':
+        import nipet_amd as nip
+        return nip.get_cylinder(params['Cnt'], rad=20., unival=1, gpu_dim=True) > 0.9
+    mask = np.ones((320, 320, 127), bool)
+    mask[:, :, :3] = False
+    return mask
+
+
+@pytest.mark.parametrize('kind', ['planes', 'disc'])
+def test_osem_mini_vs_oracle(torch, params, subsets, kind):
     from oracle import oracle as O
 
     from nipet_amd.prj.osem import osem_prepare, osem_run
     mini = _mini(params, subsets)
     rows_all = np.concatenate(mini)
     y, a, n, r = _data(params, rows_all)
-    mask = np.ones((320, 320, 127), bool)
-    mask[:, :, :3] = False
+    mask = _mask(params, kind)
     st = osem_prepare(params, y, a, n, r, None, subsets=mini, mask=mask)
     gpu = osem_run(st, 2).cpu().numpy().astype(np.float64)
     an = a.astype(np.float64) * n
@@ -71,8 +82,43 @@ def test_osem_mini_vs_oracle(torch, params, subsets):
     sens = np.stack([O.bprj(params, an[s], s) for s in mini])
     ref = O.osem(params, y, rho, sens, mask, mini, np.ones((320, 320, 127)), 2)
     rel = np.linalg.norm(gpu - ref) / np.linalg.norm(ref)
+    print(f'parity osem mini ({kind} mask): {rel:.3g} rel L2 (bar 1e-4)', flush=True)
     assert rel < 1e-4, rel
     assert np.all(gpu[~mask] == 0)
+
+
+def test_osem_window_matches_host(torch, params):
+    """nipet_osem_window: per bin, the first and the number of segments of its
+    transaxial list whose columns hold a mask voxel (columns dilated by a box
+    of half-width 8 for resolution modelling), equal to a NumPy scan of the
+    same lists."""
+    from scipy.ndimage import maximum_filter
+
+    import nipet_amd as nip
+    from nipet_amd.geom import get_geom
+    from nipet_amd.prj.osem import mask_window
+    g = get_geom(params)
+    off, col = g.tx_lists()[:2]
+    off = np.asarray(off, np.int64)
+    for rad, psf in ((29., False), (29., True), (12.5, False)):
+        m = nip.get_cylinder(params['Cnt'], rad=rad, unival=1, gpu_dim=True) > 0.9
+        m2 = m.any(axis=2)
+        if psf:
+            m2 = maximum_filter(m2.astype(np.uint8), size=17, mode='constant') > 0
+        inm = m2.ravel()[np.asarray(col)]
+        want = np.zeros((g.naw, 2), np.int64)
+        for aw in range(g.naw):
+            k = np.flatnonzero(inm[off[aw]:off[aw + 1]])
+            if k.size:
+                want[aw] = (k[0], k[-1] - k[0] + 1)
+        got = mask_window(g, torch.from_numpy(m.astype(np.uint8)).cuda(), psf).cpu().numpy()
+        assert np.array_equal(got, want), (rad, psf)
+        kept = want[:, 1].sum() / (off[1:] - off[:-1]).sum()
+        print(f'mask radius {rad} cm, psf {psf}: windows keep {kept:.3f} of the segments',
+              flush=True)
+        # the lists end at the scanner's field of view (about the 29 cm disc):
+        # only smaller masks shorten them
+        assert kept <= 1.0 and (rad > 20. or kept < 0.5)
 
 
 def test_osem_span1_mini_vs_oracle(torch, params, subsets):
@@ -216,9 +262,11 @@ def test_psf_conv_vs_oracle(torch, params):
         assert np.abs(gb[..., b] - rb).max() <= 1e-6 * np.abs(rb).max()
 
 
-def test_osem_psf_mini_vs_oracle(torch, params, subsets):
+@pytest.mark.parametrize('kind', ['planes', 'disc'])
+def test_osem_psf_mini_vs_oracle(torch, params, subsets, kind):
     """OSEM with resolution modelling (convolved sensitivity, H x projected,
-    back projection convolved) against the FP64 oracle: 1e-4 relative L2."""
+    back projection convolved) against the FP64 oracle: 1e-4 relative L2 (with
+    a disc mask the segment windows are dilated by the kernel's reach)."""
     from oracle import oracle as O
 
     from nipet_amd.prj.mmrrec import psf_config
@@ -226,7 +274,7 @@ def test_osem_psf_mini_vs_oracle(torch, params, subsets):
     mini = _mini(params, subsets)
     rows_all = np.concatenate(mini)
     y, a, n, r = _data(params, rows_all, seed=3)
-    mask = np.ones((320, 320, 127), bool)
+    mask = _mask(params, kind)
     k = psf_config(4.0, params['Cnt'])
     st = osem_prepare(params, y, a, n, r, None, subsets=mini, mask=mask, psf=k)
     gpu = osem_run(st, 2).cpu().numpy().astype(np.float64)
@@ -235,6 +283,7 @@ def test_osem_psf_mini_vs_oracle(torch, params, subsets):
     sens = np.stack([O.psf_conv(O.bprj(params, an[s], s), k) for s in mini])
     ref = O.osem(params, y, rho, sens, mask, mini, np.ones((320, 320, 127)), 2, krnl=k)
     rel = np.linalg.norm(gpu - ref) / np.linalg.norm(ref)
+    print(f'parity osem psf mini ({kind} mask): {rel:.3g} rel L2 (bar 1e-4)', flush=True)
     assert rel < 1e-4, rel
 
 
@@ -254,7 +303,7 @@ def test_petprj_osem_one_call_vs_oracle(torch, params, subsets, psf):
     y, a, n, r = _data(params, rows_all, seed=3)
     sc = np.zeros_like(r)
     sc[rows_all] = np.random.default_rng(4).uniform(0.0, 0.3, (len(rows_all), 837))
-    mask = np.ones((320, 320, 127), bool)
+    mask = _mask(params, 'disc')
     mask[:, :, 120:] = False
     an = a.astype(np.float64) * n
     rs = r.astype(np.float64) + sc
