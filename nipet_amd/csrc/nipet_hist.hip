@@ -719,16 +719,34 @@ __global__ void k_lm_ssrb(const uint32_t *psino, int nrep, const int32_t *s_off,
 constexpr int LM_FAN = 64 * NCRS;
 constexpr int LM_FAN_WG = 256;
 constexpr int LM_FAN_SUB = 4;
+// COUNT: the same walk also does k_lm_bin_count's counting (buckets in the
+// dynamic LDS, coarse buckets and a 16-bit bin table in static LDS): one read
+// of the stream instead of two, for the spans whose buckets fit beside the
+// fan-sums (span 11 and SSRB; not span 1's 21,609 buckets)
+constexpr int LM_FAN_NBK_MAX = 6144;
 
+template <bool COUNT>
 __global__ __launch_bounds__(LM_THREADS * LM_FAN_SUB) void k_lm_fan(LmDev p, int64_t nblk,
-                                                                   uint32_t *part) {
+                                                                   uint32_t *part, int nbk,
+                                                                   uint32_t *gcnt,
+                                                                   uint32_t *gccnt) {
+  extern __shared__ uint32_t bcnt[];  // COUNT: events per bucket
   __shared__ uint32_t fan[LM_FAN];
   __shared__ uint32_t cdf[LM_NCDF];
   __shared__ int32_t wave_last[LM_FAN_SUB][LM_WAVES];
+  __shared__ uint32_t ccnt[COUNT ? LM_MAXCB : 1];
+  __shared__ uint16_t lut16[COUNT ? NSN1 : 1];  // output sinogram of each span-1 sinogram
+  constexpr int NT = LM_THREADS * LM_FAN_SUB;
   const int tid = threadIdx.x, lane = tid & 63, sub = tid / LM_THREADS;
   const int wave = (tid / 64) % LM_WAVES;
-  for (int i = tid; i < LM_FAN; i += LM_THREADS * LM_FAN_SUB) fan[i] = 0;
+  for (int i = tid; i < LM_FAN; i += NT) fan[i] = 0;
   if (tid < LM_NCDF) cdf[tid] = p.cdf[tid];
+  if constexpr (COUNT) {
+    for (int i = tid; i < nbk; i += NT) bcnt[i] = 0;
+    for (int i = tid; i < p.ncb; i += NT) ccnt[i] = 0;
+    for (int i = tid; i < NSN1; i += NT)
+      lut16[i] = (uint16_t)(p.span == 11 ? p.lut[i] >> 16 : p.lut[i] & 0xffff);
+  }
   __syncthreads();
   uint32_t c8[LM_FASTK];
 #pragma unroll
@@ -754,8 +772,18 @@ __global__ __launch_bounds__(LM_THREADS * LM_FAN_SUB) void k_lm_fan(LmDev p, int
       const int32_t word = wd[j];
       const int32_t ms = lm_tag_at(word, c, le);
       const int itag = (ms < 0 ? 0 : ms - p.toff) / ITIME;
-      const bool del = word > 0 && !((word >> 30) & 1) && itag >= p.tstart && itag < p.tstop &&
+      const bool cnt = word > 0 && itag >= p.tstart && itag < p.tstop &&
                        (word & 0x3fffffff) < NSN1 * NSBINANG;
+      const bool del = cnt && !((word >> 30) & 1);
+      if constexpr (COUNT) {
+        if (cnt) {  // the events k_lm_hist stages (k_lm_bin_count's test and bin)
+          const int32_t val = word & 0x3fffffff;
+          const int si = val / NSBINANG, aw = val - si * NSBINANG;
+          const int64_t bin = (int64_t)lut16[si] * NSBINANG + aw;
+          atomicAdd(&bcnt[bin >> LM_BSHIFT], 1u);
+          atomicAdd(&ccnt[bin >> LM_CSHIFT], 1u);
+        }
+      }
       uint32_t N = 1;
       if (p.btp == 2) {  // the event's weight, as k_lm_hist draws it
         const uint64_t g = (uint64_t)(p.base + w0 + j * 64 + lane);
@@ -774,8 +802,13 @@ __global__ __launch_bounds__(LM_THREADS * LM_FAN_SUB) void k_lm_fan(LmDev p, int
     }
   }
   __syncthreads();
-  for (int i = tid; i < LM_FAN; i += LM_THREADS * LM_FAN_SUB)
-    part[(int64_t)blockIdx.x * LM_FAN + i] = fan[i];
+  for (int i = tid; i < LM_FAN; i += NT) part[(int64_t)blockIdx.x * LM_FAN + i] = fan[i];
+  if constexpr (COUNT) {
+    for (int i = tid; i < nbk; i += NT)
+      if (bcnt[i]) atomicAdd(gcnt + i, bcnt[i]);
+    for (int i = tid; i < p.ncb; i += NT)
+      if (ccnt[i]) atomicAdd(gccnt + i, ccnt[i]);
+  }
 }
 
 __global__ void k_lm_fan_sum(const uint32_t *__restrict__ part, int nwg, uint32_t *fan) {
@@ -1382,8 +1415,16 @@ int nipet_lm_chunk(NipetLm *h, const int32_t *words, int64_t n, const NipetLmOut
   } else if (bin) {
     k_lm_zero_u32<<<(h->nbk + 255) / 256, 256, 0, st>>>(h->gcnt, h->nbk);
     k_lm_zero_u32<<<(h->ncb + 255) / 256, 256, 0, st>>>(h->gccnt, h->ncb);
-    k_lm_bin_count<<<(unsigned)std::min<int64_t>(nblk, 2048), LM_THREADS,
-                     (size_t)h->nbk * sizeof(uint32_t), st>>>(p, nblk, h->nbk, h->gcnt, h->gccnt);
+    const int nwg = (int)std::min<int64_t>((nblk + LM_FAN_SUB - 1) / LM_FAN_SUB, LM_FAN_WG);
+    const bool merged = h->nbk <= LM_FAN_NBK_MAX;  // counts in the fan-sum walk
+    if (merged) {
+      k_lm_fan<true><<<(unsigned)nwg, LM_THREADS * LM_FAN_SUB, (size_t)h->nbk * sizeof(uint32_t),
+                       st>>>(p, nblk, h->fan_part, h->nbk, h->gcnt, h->gccnt);
+    } else {
+      k_lm_bin_count<<<(unsigned)std::min<int64_t>(nblk, 2048), LM_THREADS,
+                       (size_t)h->nbk * sizeof(uint32_t), st>>>(p, nblk, h->nbk, h->gcnt,
+                                                                 h->gccnt);
+    }
     k_lm_bin_scan<<<1, 1024, 0, st>>>(h->gcnt, h->nbk, h->bstart, h->bcur);
     k_lm_bin_scan<<<1, 1024, 0, st>>>(h->gccnt, h->ncb, h->cstart, h->ccur);
     k_lm_hist<false, false, true><<<(unsigned)nblk, LM_THREADS, 0, st>>>(p);
@@ -1391,8 +1432,9 @@ int nipet_lm_chunk(NipetLm *h, const int32_t *words, int64_t n, const NipetLmOut
         h->stage, h->cstart, h->ncb, h->nbk, h->bcur, h->stage2);
     k_lm_bin_apply<<<(unsigned)h->nbk, 1024, 0, st>>>(h->stage2, h->bstart, h->nbins, o->psino,
                                                        o->dsino);
-    const int nwg = (int)std::min<int64_t>((nblk + LM_FAN_SUB - 1) / LM_FAN_SUB, LM_FAN_WG);
-    k_lm_fan<<<(unsigned)nwg, LM_THREADS * LM_FAN_SUB, 0, st>>>(p, nblk, h->fan_part);
+    if (!merged)
+      k_lm_fan<false><<<(unsigned)nwg, LM_THREADS * LM_FAN_SUB, 0, st>>>(p, nblk, h->fan_part, 0,
+                                                                         nullptr, nullptr);
     k_lm_fan_sum<<<(LM_FAN + 255) / 256, 256, 0, st>>>(h->fan_part, nwg, o->fan);
   } else if (c.nrep == 1) {
     k_lm_hist<false, false><<<(unsigned)nblk, LM_THREADS, 0, st>>>(p);
