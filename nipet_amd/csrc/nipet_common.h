@@ -50,7 +50,16 @@ struct Geom {
   // of one ring difference; fp_slot[s] = d * 64 + r0
   int16_t *fp_slot = nullptr;
   int fp_nslot = 0;
+  // back-projector LOR tables [2][BP_TABN]: output sinogram of LOR (r0, d)
+  // and of its mirror at bp_row(d) + d + 1 + r0, 0xffff where there is none
+  uint16_t *bp_tab = nullptr;
 };
+
+// Rows of the back projector's LOR tables: row d holds r0 = -d-1 .. 64
+// (66 + d entries), so that the gather's r0 - h and r0 - e stay in the row.
+constexpr int BP_MAXD = 63;  // largest ring difference of the BP tables
+__host__ __device__ constexpr int bp_row(int d) { return 66 * d + d * (d - 1) / 2; }
+constexpr int BP_TABN = bp_row(BP_MAXD + 1);
 
 constexpr int FP_G = 3;          // LORs per forward-projector lane
 constexpr int FP_G_SLOTS = 768;  // lanes of that kernel

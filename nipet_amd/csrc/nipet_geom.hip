@@ -275,7 +275,7 @@ int nipet_prof_read(int kernel, double *total_ms, int64_t *count) {
 static void geom_free(Geom *g) {
   // teardown: a failing free leaves nothing to undo
   void *bufs[] = {g->off,   g->seg,    g->atn,    g->lmax,   g->li_z,   g->li_sn,
-                  g->li_nosi, g->dr_li, g->sn_off, g->sn_mem, g->fp_slot};
+                  g->li_nosi, g->dr_li, g->sn_off, g->sn_mem, g->fp_slot, g->bp_tab};
   for (void *p : bufs) (void)hipFree(p);
 }
 
@@ -377,6 +377,19 @@ static int geom_build(const NipetGeomDesc *d, NipetGeom *g) {
     }
     NIPET_HIP(hipMalloc(&g->fp_slot, slots.size() * sizeof(int16_t)));
     NIPET_HIP(hipMemcpy(g->fp_slot, slots.data(), slots.size() * sizeof(int16_t),
+                        hipMemcpyHostToDevice));
+  }
+  {
+    // back-projector LOR tables (nsn < 0xffff: 0xffff marks "no LOR")
+    std::vector<uint16_t> tab(2 * (size_t)BP_TABN, 0xffff);
+    for (int li = 0; li < d->nli; li++) {
+      const int r0 = (int)floorf(liz[li].x) >> 1, dd = (int)liz[li].y >> 1;
+      const int k = bp_row(dd) + dd + 1 + r0;
+      tab[k] = (uint16_t)lisn[li].x;
+      if (dd > 0) tab[BP_TABN + k] = (uint16_t)lisn[li].y;
+    }
+    NIPET_HIP(hipMalloc(&g->bp_tab, tab.size() * sizeof(uint16_t)));
+    NIPET_HIP(hipMemcpy(g->bp_tab, tab.data(), tab.size() * sizeof(uint16_t),
                         hipMemcpyHostToDevice));
   }
   NIPET_HIP(hipMalloc(&g->dr_li, drli.size() * sizeof(int16_t)));

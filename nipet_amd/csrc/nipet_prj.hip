@@ -28,13 +28,10 @@ constexpr int BP_THREADS = 512;   // BP: 8 waves
 constexpr int SUBSUM = 8;         // float partial sums over 8 segments
 constexpr int ZSTRIDE = 128;      // LDS z-column (127 + zero pad)
 constexpr int MAXNB = 4;          // replicates per kernel instance (larger batches: sub-batches)
-constexpr int BP_MAXD = 63;       // largest ring difference of the BP tables
 
 // BP table row d holds r0 = -2 .. 65-d (two zero pads on both sides); row offset:
 // Row d of the BP index tables covers r0 = -d-1 .. 64 (d + 66 entries), every
 // index the gather can form for ring difference d, so no clamping is needed.
-__host__ __device__ constexpr int bp_row(int d) { return 66 * d + d * (d - 1) / 2; }
-constexpr int BP_TABN = bp_row(BP_MAXD + 1);
 
 struct GeomView {
   const int32_t *off;
@@ -48,6 +45,7 @@ struct GeomView {
   const int32_t *sn_off;
   const int16_t *sn_mem;
   const int16_t *fp_slot;
+  const uint16_t *bp_tab;
   int fp_nslot;
   int naw, nli, ndirect, nsn, dmax;
   float vox_z;
@@ -66,6 +64,7 @@ static GeomView view(const Geom *g) {
   v.sn_off = g->sn_off;
   v.sn_mem = g->sn_mem;
   v.fp_slot = g->fp_slot;
+  v.bp_tab = g->bp_tab;
   v.fp_nslot = g->fp_nslot;
   v.naw = g->naw;
   v.nli = g->nli;
@@ -804,7 +803,6 @@ __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g
   constexpr int NWAVE = BP_THREADS / 64;
   __shared__ __align__(16) float srow[(MAXNSN / NB + 1) * NB];   // sinogram row + zero row
   __shared__ __align__(16) float wbuf[NWAVE][ZSTRIDE * NB];
-  __shared__ int dmax_s;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int row = blockIdx.x;
   const int aw = subs ? subs[row] : row;
@@ -824,26 +822,16 @@ __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g
     const int s = i / NB, b = i - s * NB;
     srow[i] = s < nsn ? in[(size_t)s * nbt + b] : 0.f;
   }
+  // (built once with the geometry; here scaled to byte offsets)
   __shared__ uint16_t ta[BP_TABN], tb[BP_TABN];
-  const uint16_t zoff = (uint16_t)(nsn * NB * 4);
   for (int i = tid; i < BP_TABN; i += BP_THREADS) {
-    ta[i] = zoff;
-    tb[i] = zoff;
-  }
-  if (tid == 0) dmax_s = 0;
-  __syncthreads();
-  for (int li = tid; li < g.nli; li += BP_THREADS) {
-    float2 z = g.li_z[li];
-    short2 sn = g.li_sn[li];
-    int r0 = (int)floorf(z.x) >> 1, d = (int)z.y >> 1;
-    int k = bp_row(d) + d + 1 + r0;
-    ta[k] = (uint16_t)(sn.x * NB * 4);
-    if (d > 0) tb[k] = (uint16_t)(sn.y * NB * 4);
-    atomicMax(&dmax_s, d);
+    const int a = g.bp_tab[i], b = g.bp_tab[BP_TABN + i];
+    ta[i] = (uint16_t)((a == 0xffff ? nsn : a) * NB * 4);
+    tb[i] = (uint16_t)((b == 0xffff ? nsn : b) * NB * 4);
   }
   __syncthreads();
   const char *srb = (const char *)srow;
-  const int dmax = dmax_s;
+  const int dmax = g.dmax;
   const double m_lane = 2.0 * lane / atn;
   const double inv_lane = lane > 0 ? atn / (2.0 * lane) : 0.0;
   const float sf_lane = lor_sfac(2.f * lane, g.vox_z, atn);
