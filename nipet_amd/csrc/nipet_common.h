@@ -50,8 +50,10 @@ struct Geom {
   // of one ring difference; fp_slot[s] = d * 64 + r0
   int16_t *fp_slot = nullptr;
   int fp_nslot = 0;
-  // back-projector LOR tables [2][BP_TABN]: output sinogram of LOR (r0, d)
-  // and of its mirror at bp_row(d) + d + 1 + r0, 0xffff where there is none
+  // back-projector LOR tables [3][2][BP_TABN] for 1, 2 and 4 images: byte
+  // offset into the back projector's LDS sinogram row (sinogram * nb * 4) of
+  // LOR (r0, d) and of its mirror at bp_row(d) + d + 1 + r0; the zero row
+  // (nsn * nb * 4) where there is none
   uint16_t *bp_tab = nullptr;
 };
 

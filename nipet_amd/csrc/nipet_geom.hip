@@ -380,13 +380,24 @@ static int geom_build(const NipetGeomDesc *d, NipetGeom *g) {
                         hipMemcpyHostToDevice));
   }
   {
-    // back-projector LOR tables (nsn < 0xffff: 0xffff marks "no LOR")
-    std::vector<uint16_t> tab(2 * (size_t)BP_TABN, 0xffff);
-    for (int li = 0; li < d->nli; li++) {
-      const int r0 = (int)floorf(liz[li].x) >> 1, dd = (int)liz[li].y >> 1;
-      const int k = bp_row(dd) + dd + 1 + r0;
-      tab[k] = (uint16_t)lisn[li].x;
-      if (dd > 0) tab[BP_TABN + k] = (uint16_t)lisn[li].y;
+    // back-projector LOR tables, scaled for 1, 2 and 4 images: 16-bit byte
+    // offsets into a row of nsn + 1 sinograms of 4 images
+    if (d->nsn * 4 * 4 > 0xffff) {
+      set_error("%d sinograms: the back projector's 16-bit row offsets reach 4095", d->nsn);
+      return NIPET_EINVAL;
+    }
+    std::vector<uint16_t> tab(6 * (size_t)BP_TABN);
+    const int nbs[3] = {1, 2, 4};
+    for (int w = 0; w < 3; w++) {
+      uint16_t *t = tab.data() + (size_t)w * 2 * BP_TABN;
+      const uint16_t zr = (uint16_t)(d->nsn * nbs[w] * 4);
+      for (int i = 0; i < 2 * BP_TABN; i++) t[i] = zr;
+      for (int li = 0; li < d->nli; li++) {
+        const int r0 = (int)floorf(liz[li].x) >> 1, dd = (int)liz[li].y >> 1;
+        const int k = bp_row(dd) + dd + 1 + r0;
+        t[k] = (uint16_t)(lisn[li].x * nbs[w] * 4);
+        if (dd > 0) t[BP_TABN + k] = (uint16_t)(lisn[li].y * nbs[w] * 4);
+      }
     }
     NIPET_HIP(hipMalloc(&g->bp_tab, tab.size() * sizeof(uint16_t)));
     NIPET_HIP(hipMemcpy(g->bp_tab, tab.data(), tab.size() * sizeof(uint16_t),

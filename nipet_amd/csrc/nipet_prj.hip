@@ -822,12 +822,16 @@ __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g
     const int s = i / NB, b = i - s * NB;
     srow[i] = s < nsn ? in[(size_t)s * nbt + b] : 0.f;
   }
-  // (built once with the geometry; here scaled to byte offsets)
-  __shared__ uint16_t ta[BP_TABN], tb[BP_TABN];
-  for (int i = tid; i < BP_TABN; i += BP_THREADS) {
-    const int a = g.bp_tab[i], b = g.bp_tab[BP_TABN + i];
-    ta[i] = (uint16_t)((a == 0xffff ? nsn : a) * NB * 4);
-    tb[i] = (uint16_t)((b == 0xffff ? nsn : b) * NB * 4);
+  // (built once with the geometry for this width; copied in 16-byte pieces)
+  static_assert((2 * BP_TABN * sizeof(uint16_t)) % 16 == 0, "tables copy as uint4");
+  __shared__ __align__(16) uint16_t tab[2 * BP_TABN];
+  const uint16_t *ta = tab, *tb = tab + BP_TABN;
+  {
+    const uint4 *src = (const uint4 *)(g.bp_tab + (size_t)(NB == 1 ? 0 : (NB == 2 ? 1 : 2)) * 2 *
+                                                      BP_TABN);
+    uint4 *dst = (uint4 *)tab;
+    for (int i = tid; i < 2 * BP_TABN * (int)sizeof(uint16_t) / 16; i += BP_THREADS)
+      dst[i] = src[i];
   }
   __syncthreads();
   const char *srb = (const char *)srow;
