@@ -274,6 +274,20 @@ __device__ inline void fp_walk(FpLor &l, FpSum<NB> &acc, const float2 *__restric
   }
 }
 
+// The output sinograms' LOR lists (CSR sn_off / sn_mem, at most 2 LI_MAX
+// members < 65536) copied into LDS at the start of a bin, so that the final
+// per-sinogram sums read LDS instead of chained global loads.
+struct SnCsr {
+  uint16_t off[MAXNSN + 1];
+  uint16_t mem[2 * LI_MAX];
+};
+
+__device__ inline void load_sn_csr(SnCsr &c, const GeomView &g, int tid, int nthreads) {
+  for (int i = tid; i <= g.nsn; i += nthreads) c.off[i] = (uint16_t)g.sn_off[i];
+  const int nm = g.sn_off[g.nsn];
+  for (int i = tid; i < nm; i += nthreads) c.mem[i] = (uint16_t)g.sn_mem[i];
+}
+
 template <int MODE, int NB>
 __global__ __launch_bounds__(FP_THREADS, NB >= 4 ? 4 : 6) void k_fprj(GeomView g, const float *__restrict__ img,
                                                      float *__restrict__ sino,
@@ -302,6 +316,8 @@ __global__ __launch_bounds__(FP_THREADS, NB >= 4 ? 4 : 6) void k_fprj(GeomView g
   const double atn = g.atn[aw];
   const float lmax = g.lmax[aw];
   float *zs = lds;
+  __shared__ SnCsr csr;
+  load_sn_csr(csr, g, tid, FP_THREADS);
   // guard planes (never written by the staging loop)
   for (int i = tid; i < FP_GUARD * PS * NB; i += FP_THREADS) {
     zs[i] = 0.f;
@@ -408,8 +424,8 @@ __global__ __launch_bounds__(FP_THREADS, NB >= 4 ? 4 : 6) void k_fprj(GeomView g
     __syncthreads();
     for (int s = tid; s < nsn; s += FP_THREADS) {
       float v = 0.f;
-      const int m1 = g.sn_off[s + 1];
-      for (int m = g.sn_off[s]; m < m1; m++) v += lorv[g.sn_mem[m]];
+      const int m1 = csr.off[s + 1];
+      for (int m = csr.off[s]; m < m1; m++) v += lorv[csr.mem[m]];
       if (MODE == 2) {
         const size_t kk = (size_t)aw * nsn + s;
         const float ee = v + (rho_nb ? rho[kk * nbt + b0 + b] : rho[kk]);
@@ -569,6 +585,8 @@ __global__ __launch_bounds__(fpg_threads<G>()) void k_fprj_g(GeomView g, const f
   const double atn = g.atn[aw];
   const float lmax = g.lmax[aw];
   float *zs = lds;
+  __shared__ SnCsr csr;
+  load_sn_csr(csr, g, tid, FPG_THREADS);
   // guard planes (never written by the staging loop)
   for (int i = tid; i < FP_GUARD * PS * NB; i += FPG_THREADS) zs[i] = 0.f;
   for (int i = tid; i < fpg_top<G>() * PS * NB; i += FPG_THREADS)
@@ -696,8 +714,8 @@ __global__ __launch_bounds__(fpg_threads<G>()) void k_fprj_g(GeomView g, const f
     __syncthreads();
     for (int s = tid; s < nsn; s += FPG_THREADS) {
       float v = 0.f;
-      const int m1 = g.sn_off[s + 1];
-      for (int m = g.sn_off[s]; m < m1; m++) v += lorv[g.sn_mem[m]];
+      const int m1 = csr.off[s + 1];
+      for (int m = csr.off[s]; m < m1; m++) v += lorv[csr.mem[m]];
       if (MODE == 2) {
         const size_t kk = (size_t)aw * nsn + s;
         const float ee = v + (rho_nb ? rho[kk * nbt + b0 + b] : rho[kk]);
