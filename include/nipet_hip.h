@@ -287,7 +287,9 @@ typedef struct {
 } NipetLmOut;
 
 int nipet_lm_create(const NipetLmConfig *cfg, NipetLm **out);
-/* Histogram words[0..n) (device int32) on `stream`. */
+/* Histogram words[0..n) (device int32) on `stream`. A chunk that fails after
+ * its first kernels were queued (a device allocation for the staging slots)
+ * leaves the session unusable: later chunk / finish calls return an error. */
 int nipet_lm_chunk(NipetLm *h, const int32_t *words, int64_t n, const NipetLmOut *out,
                    void *stream);
 /* After the last chunk: the SSRB prompt sinogram out->ssr (overwritten) from

@@ -1091,6 +1091,9 @@ struct NipetLm {
   uint32_t *stage = nullptr, *stage2 = nullptr;  // coarse order, bucket order
   int64_t stage_cap = 0;
   uint32_t *fan_part = nullptr;  // [LM_FAN_WG][64 x 504] fan-sum copies
+  // set when a chunk failed after its first kernels had advanced the
+  // session's time-tag carry: the session can no longer be continued
+  bool broken = false;
 };
 
 static void lm_free(NipetLm *h) {
@@ -1301,7 +1304,22 @@ void nipet_lm_destroy(NipetLm *h) {
   delete h;
 }
 
-static int grow(NipetLm *h, int64_t nblk, unsigned int sgl, int64_t stage) {
+// Singles list and the binned path's staging slots (one per word of the
+// chunk). Allocated after the chunk's first kernels are queued, so that the
+// host-side cost of these large allocations overlaps their run time.
+static int grow_late(NipetLm *h, unsigned int sgl, int64_t stage) {
+  if (sgl > h->sgl_cap) {
+    (void)hipFree(h->sgl_idx);
+    (void)hipFree(h->sgl_word);
+    (void)hipFree(h->sgl_ms);
+    h->sgl_idx = nullptr;
+    h->sgl_word = h->sgl_ms = nullptr;
+    h->sgl_cap = 0;
+    NIPET_HIP(hipMalloc(&h->sgl_idx, (size_t)sgl * sizeof(unsigned long long)));
+    NIPET_HIP(hipMalloc(&h->sgl_word, (size_t)sgl * sizeof(int32_t)));
+    NIPET_HIP(hipMalloc(&h->sgl_ms, (size_t)sgl * sizeof(int32_t)));
+    h->sgl_cap = sgl;
+  }
   if (stage > h->stage_cap) {
     (void)hipFree(h->stage);
     (void)hipFree(h->stage2);
@@ -1311,21 +1329,18 @@ static int grow(NipetLm *h, int64_t nblk, unsigned int sgl, int64_t stage) {
     NIPET_HIP(hipMalloc(&h->stage2, (size_t)stage * sizeof(uint32_t)));
     h->stage_cap = stage;
   }
+  return NIPET_OK;
+}
+
+static int grow(NipetLm *h, int64_t nblk) {
   if (nblk > h->blk_cap) {
     (void)hipFree(h->blk_last);
     (void)hipFree(h->blk_carry);
+    h->blk_last = h->blk_carry = nullptr;
+    h->blk_cap = 0;
     NIPET_HIP(hipMalloc(&h->blk_last, nblk * sizeof(int32_t)));
     NIPET_HIP(hipMalloc(&h->blk_carry, nblk * sizeof(int32_t)));
     h->blk_cap = nblk;
-  }
-  if (sgl > h->sgl_cap) {
-    (void)hipFree(h->sgl_idx);
-    (void)hipFree(h->sgl_word);
-    (void)hipFree(h->sgl_ms);
-    NIPET_HIP(hipMalloc(&h->sgl_idx, (size_t)sgl * sizeof(unsigned long long)));
-    NIPET_HIP(hipMalloc(&h->sgl_word, (size_t)sgl * sizeof(int32_t)));
-    NIPET_HIP(hipMalloc(&h->sgl_ms, (size_t)sgl * sizeof(int32_t)));
-    h->sgl_cap = sgl;
   }
   return NIPET_OK;
 }
@@ -1339,6 +1354,10 @@ int nipet_lm_chunk(NipetLm *h, const int32_t *words, int64_t n, const NipetLmOut
   if (!o->psino || !o->dsino || !o->ssr || !o->phc || !o->dhc || !o->zR || !o->zM || !o->pvs ||
       !o->fan || !o->bck) {
     set_error("nipet_lm_chunk: every output buffer is required");
+    return NIPET_EINVAL;
+  }
+  if (h->broken) {
+    set_error("nipet_lm_chunk: an earlier chunk of this session failed part-way");
     return NIPET_EINVAL;
   }
   if (n == 0) return NIPET_OK;
@@ -1363,12 +1382,12 @@ int nipet_lm_chunk(NipetLm *h, const int32_t *words, int64_t n, const NipetLmOut
   // one replicate without resampling: sinogram counts privatised in LDS per
   // bucket (staged events need a slot per word of the chunk)
   const bool bin = h->gcnt && n <= (int64_t)UINT32_MAX;
-  int rc = grow(h, nblk, cap, bin ? n : 0);
+  int rc = grow(h, nblk);
   if (rc) return rc;
   prof_begin(PK_HIST, st);
   k_lm_tags<<<(unsigned)nblk, LM_THREADS, 0, st>>>(words, n, h->blk_last, h->sgl_count);
   k_lm_carry<<<1, 1024, 0, st>>>(h->blk_last, h->blk_carry, (int)nblk, h->carry);
-  LmDev p;
+  LmDev p = {};  // singles list and staging slots are set once allocated
   p.words = words;
   p.n = n;
   p.base = h->base;
@@ -1397,26 +1416,17 @@ int nipet_lm_chunk(NipetLm *h, const int32_t *words, int64_t n, const NipetLmOut
   p.fan = o->fan;
   p.zR = (unsigned long long *)o->zR;
   p.zM = (unsigned long long *)o->zM;
-  p.sgl_idx = h->sgl_idx;
-  p.sgl_word = h->sgl_word;
-  p.sgl_ms = h->sgl_ms;
   p.sgl_count = h->sgl_count;
-  p.sgl_cap = h->sgl_cap;
   p.bad = h->bad;
   p.ccur = h->ccur;
-  p.stage = h->stage;
-  p.stage_cap = bin ? n : 0;
   p.ncb = h->ncb;
-  if (c.btp == 1) {
-    if (c.nrep == 1)
-      k_lm_hist<false, true><<<(unsigned)nblk, LM_THREADS, 0, st>>>(p);
-    else
-      k_lm_hist<true, true><<<(unsigned)nblk, LM_THREADS, 0, st>>>(p);
-  } else if (bin) {
+  // binned path: bucket counts (with the fan-sums where they fit) and their
+  // scans go first; they read the words and the tag carry only
+  const int nwg = (int)std::min<int64_t>((nblk + LM_FAN_SUB - 1) / LM_FAN_SUB, LM_FAN_WG);
+  const bool merged = h->nbk <= LM_FAN_NBK_MAX;  // counts in the fan-sum walk
+  if (bin) {
     k_lm_zero_u32<<<(h->nbk + 255) / 256, 256, 0, st>>>(h->gcnt, h->nbk);
     k_lm_zero_u32<<<(h->ncb + 255) / 256, 256, 0, st>>>(h->gccnt, h->ncb);
-    const int nwg = (int)std::min<int64_t>((nblk + LM_FAN_SUB - 1) / LM_FAN_SUB, LM_FAN_WG);
-    const bool merged = h->nbk <= LM_FAN_NBK_MAX;  // counts in the fan-sum walk
     if (merged) {
       k_lm_fan<true><<<(unsigned)nwg, LM_THREADS * LM_FAN_SUB, (size_t)h->nbk * sizeof(uint32_t),
                        st>>>(p, nblk, h->fan_part, h->nbk, h->gcnt, h->gccnt);
@@ -1427,6 +1437,25 @@ int nipet_lm_chunk(NipetLm *h, const int32_t *words, int64_t n, const NipetLmOut
     }
     k_lm_bin_scan<<<1, 1024, 0, st>>>(h->gcnt, h->nbk, h->bstart, h->bcur);
     k_lm_bin_scan<<<1, 1024, 0, st>>>(h->gccnt, h->ncb, h->cstart, h->ccur);
+  }
+  // the tag carry has advanced: a failure from here on ends the session
+  rc = grow_late(h, cap, bin ? n : 0);
+  if (rc) {
+    h->broken = true;
+    return rc;
+  }
+  p.sgl_idx = h->sgl_idx;
+  p.sgl_word = h->sgl_word;
+  p.sgl_ms = h->sgl_ms;
+  p.sgl_cap = h->sgl_cap;
+  p.stage = bin ? h->stage : nullptr;
+  p.stage_cap = bin ? n : 0;
+  if (c.btp == 1) {
+    if (c.nrep == 1)
+      k_lm_hist<false, true><<<(unsigned)nblk, LM_THREADS, 0, st>>>(p);
+    else
+      k_lm_hist<true, true><<<(unsigned)nblk, LM_THREADS, 0, st>>>(p);
+  } else if (bin) {
     k_lm_hist<false, false, true><<<(unsigned)nblk, LM_THREADS, 0, st>>>(p);
     k_lm_bin_part<<<(unsigned)((n + LM_TILE - 1) / LM_TILE), LM_THREADS, 0, st>>>(
         h->stage, h->cstart, h->ncb, h->nbk, h->bcur, h->stage2);
@@ -1454,6 +1483,10 @@ int nipet_lm_chunk(NipetLm *h, const int32_t *words, int64_t n, const NipetLmOut
 int nipet_lm_finish(NipetLm *h, const NipetLmOut *o, void *stream) {
   if (!h || !o || !o->psino || !o->ssr) {
     set_error("nipet_lm_finish: bad argument");
+    return NIPET_EINVAL;
+  }
+  if (h->broken) {
+    set_error("nipet_lm_finish: a chunk of this session failed part-way");
     return NIPET_EINVAL;
   }
   k_lm_ssrb<<<4096, 256, 0, (hipStream_t)stream>>>(o->psino, h->cfg.nrep, h->ssrb_off,
