@@ -448,12 +448,13 @@ __global__ __launch_bounds__(FP_THREADS, NB >= 4 ? 4 : 6) void k_fprj(GeomView g
 // arithmetic equals k_fprj's, so both kernels give the same values. Slots (d, r0) come from the
 // geometry (fp_slot); lanes whose group runs past ring 63 walk phantom LORs
 // whose planes lie in zero guard planes and are never written out.
-// G = 3 (odd): lanes of one ring difference then sit 6 planes apart, which
-// puts them on distinct LDS banks (plane stride 17 floats; an even G would fold
-// them onto 8 or 16 banks). Measured on MI355X: G = 3 1.19 ms per subset,
-// G = 5 1.31 ms, G = 4 3.4 ms (bank conflicts), one LOR per lane 1.71 ms. With
-// two replicates (plane stride 34 floats) every G conflicts and k_fprj<., 2>
-// is faster, so only single images use this kernel.
+// G = 3 (odd): lanes of one ring difference then sit 6 planes (3 plane pairs)
+// apart; in the pair layout (fpg_addr) that is 3 banks, so 32 lanes of one d
+// reach 32 distinct banks. Measured on MI355X with a linear plane stride of 17
+// floats: G = 3 1.19 ms per subset, G = 5 1.31 ms, G = 4 3.4 ms (bank
+// conflicts), one LOR per lane 1.71 ms. With two replicates (plane stride 34
+// floats) every G conflicts and k_fprj<., 2> is faster, so only single images
+// use this kernel.
 template <int G>
 constexpr int fpg_threads() { return FP_G_SLOTS; }  // >= 712 slots of 3 LORs
 template <int G>
@@ -480,15 +481,38 @@ struct FpSumG {
   }
 };
 
-template <int NB, int G, bool TWO>
-__device__ inline void fpg_walk(FpLor &l, FpSumG<NB, G> &acc, const float2 *__restrict__ sr,
+// Staged chunk of the grouped kernel: planes in pairs, plane p at
+// fpg_addr(p) = (p >> 1) * FPG_PP + (p & 1) * CH floats (segment j at + j),
+// pair stride FPG_PP = 2 CH + 1. Planes 2k above p stay at the immediate
+// offset FPG_PP * k; the next plane up is CH or CH + 1 floats on, by the
+// parity of p (kept per lane as oa / ob). Lanes of one ring difference sit 6
+// planes (3 pairs) apart at every segment, so their reads step 3 banks and
+// reach all 32 banks; a linear plane stride maps 6 planes to an even number of
+// banks, 16 of the 32.
+constexpr int FPG_CH = 16;
+constexpr int FPG_PP = 2 * FPG_CH + 1;
+__host__ __device__ constexpr int fpg_addr(int p) { return (p >> 1) * FPG_PP + (p & 1) * FPG_CH; }
+
+struct FpLorG {
+  float tbr;   // next z boundary of LOR a relative to the chunk's origin (float)
+  float invf;  // boundary spacing (float)
+  int ao;      // staged-chunk offset of plane pa = ia + FP_GUARD (LOR a), segment 0
+  int bo;      // staged-chunk offset of plane pb = ib - 2 + FP_GUARD (mirror b)
+  // fpg_addr(p + 1) - fpg_addr(p) = CH + (p & 1) for p = pa and p = pb:
+  // pa + pb = 4 r0 + 2 d + 2 is even, so both planes share their parity (and
+  // keep sharing it, since a boundary moves pa up and pb down by one)
+  int o;
+};
+
+template <int G, bool TWO>
+__device__ inline void fpg_walk(FpLorG &l, FpSumG<1, G> &acc, const float2 *__restrict__ sr,
                                 const float *__restrict__ zs, int nch) {
-  constexpr int PS = fp_ps<NB>() * NB;  // plane stride (floats)
-  constexpr int UNROLL = NB >= 2 ? 2 : 4;
+  constexpr int PP = FPG_PP;
   for (int j8 = 0; j8 < nch; j8 += SUBSUM) {
-    const float *za = zs + l.ao + j8 * NB;
-    const float *zb = zs + l.bo + j8 * NB;
-#pragma unroll UNROLL
+    const float *za = zs + l.ao + j8;
+    const float *zb = zs + l.bo + j8;
+    int o = l.o;
+#pragma unroll 4
     for (int jj = 0; jj < SUBSUM; jj++) {
       const float2 sl = sr[j8 + jj];
       if constexpr (!TWO) {
@@ -497,74 +521,71 @@ __device__ inline void fpg_walk(FpLor &l, FpSumG<NB, G> &acc, const float2 *__re
         const bool cr = q <= sl.y;
         static_for<G>([&](auto kc) {
           constexpr int k = decltype(kc)::value;
-          float xa0[NB], xb0[NB];
-          ldv<NB>(xa0, za + 2 * k * PS + jj * NB);
-          ldv<NB>(xb0, zb + (2 * k + 2) * PS + jj * NB);
-          fma_nb<NB>(acc.pa[k], p1, xa0);
-          fma_nb<NB>(acc.pb[k], p1, xb0);
+          acc.pa[k][0] = fmaf(p1, za[k * PP + jj], acc.pa[k][0]);       // plane pa + 2k
+          acc.pb[k][0] = fmaf(p1, zb[(k + 1) * PP + jj], acc.pb[k][0]);  // plane pb + 2k + 2
         });
         // second piece only where the segment holds a boundary: the same
         // segment for every lane of a ring difference, so whole lane groups
-        // skip these plane reads together
+        // skip these reads (and the plane steps) together
         if (cr) {
           const float p2 = sl.y - p1;
+          const float *za1 = za + o, *zb1 = zb + o;  // planes pa + 1, pb + 1
           static_for<G>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
-            float xa1[NB], xb1[NB];
-            ldv<NB>(xa1, za + (2 * k + 1) * PS + jj * NB);
-            ldv<NB>(xb1, zb + (2 * k + 1) * PS + jj * NB);
-            fma_nb<NB>(acc.pa[k], p2, xa1);
-            fma_nb<NB>(acc.pb[k], p2, xb1);
+            acc.pa[k][0] = fmaf(p2, za1[k * PP + jj], acc.pa[k][0]);
+            acc.pb[k][0] = fmaf(p2, zb1[k * PP + jj], acc.pb[k][0]);
           });
+          za = za1;       // pa + 1
+          zb = zb1 - PP;  // pb - 1
+          o = PP - o;
+          l.tbr += l.invf;
         }
-        const int step = cr ? PS : 0;
-        za += step;
-        zb -= step;
-        l.tbr += cr ? l.invf : 0.f;
       } else {
         const float q1 = l.tbr - sl.x, q2 = q1 + l.invf;
         const float p1 = __builtin_amdgcn_fmed3f(q1, 0.f, sl.y);
         const float c2 = __builtin_amdgcn_fmed3f(q2, 0.f, sl.y);
         const float p2 = c2 - p1, p3 = sl.y - c2;
+        const float *za1 = za + o, *zb1 = zb + o;
         static_for<G>([&](auto kc) {
           constexpr int k = decltype(kc)::value;
-          float xa0[NB], xa1[NB], xa2[NB], xb0[NB], xb1[NB], xb2[NB];
-          ldv<NB>(xa0, za + 2 * k * PS + jj * NB);
-          ldv<NB>(xa1, za + (2 * k + 1) * PS + jj * NB);
-          ldv<NB>(xa2, za + (2 * k + 2) * PS + jj * NB);
-          ldv<NB>(xb0, zb + (2 * k + 2) * PS + jj * NB);
-          ldv<NB>(xb1, zb + (2 * k + 1) * PS + jj * NB);
-          ldv<NB>(xb2, zb + 2 * k * PS + jj * NB);
-          fma_nb<NB>(acc.pa[k], p1, xa0);
-          fma_nb<NB>(acc.pa[k], p2, xa1);
-          fma_nb<NB>(acc.pa[k], p3, xa2);
-          fma_nb<NB>(acc.pb[k], p1, xb0);
-          fma_nb<NB>(acc.pb[k], p2, xb1);
-          fma_nb<NB>(acc.pb[k], p3, xb2);
+          const float xa0 = za[k * PP + jj], xa1 = za1[k * PP + jj], xa2 = za[(k + 1) * PP + jj];
+          const float xb0 = zb[(k + 1) * PP + jj], xb1 = zb1[k * PP + jj], xb2 = zb[k * PP + jj];
+          acc.pa[k][0] = fmaf(p1, xa0, acc.pa[k][0]);
+          acc.pa[k][0] = fmaf(p2, xa1, acc.pa[k][0]);
+          acc.pa[k][0] = fmaf(p3, xa2, acc.pa[k][0]);
+          acc.pb[k][0] = fmaf(p1, xb0, acc.pb[k][0]);
+          acc.pb[k][0] = fmaf(p2, xb1, acc.pb[k][0]);
+          acc.pb[k][0] = fmaf(p3, xb2, acc.pb[k][0]);
         });
         const int n = (int)(q1 <= sl.y) + (int)(q2 <= sl.y);
-        za += n * PS;
-        zb -= n * PS;
+        const bool n1 = n == 1, n2 = n == 2;
+        za = n1 ? za1 : (n2 ? za + PP : za);
+        zb = n1 ? zb1 - PP : (n2 ? zb - PP : zb);
+        o = n1 ? PP - o : o;
         l.tbr += (float)n * l.invf;
       }
     }
-    l.ao = (int)(za - zs) - j8 * NB;
-    l.bo = (int)(zb - zs) - j8 * NB;
+    l.ao = (int)(za - zs) - j8;
+    l.bo = (int)(zb - zs) - j8;
+    l.o = o;
     acc.fold();
   }
 }
 
+// 6 waves per SIMD (two 12-wave workgroups per CU) need <= 80 VGPRs; the
+// compiler's own choice for this kernel is 82
 template <int MODE, int NB, int G>
-__global__ __launch_bounds__(fpg_threads<G>()) void k_fprj_g(GeomView g, const float *__restrict__ img,
+__global__ __launch_bounds__(fpg_threads<G>()) __attribute__((amdgpu_waves_per_eu(6))) void k_fprj_g(GeomView g, const float *__restrict__ img,
                                                            float *__restrict__ sino,
                                                            const int32_t *__restrict__ subs,
                                                            const uint16_t *__restrict__ prompts,
                                                            const float *__restrict__ rho,
                                                            int rho_nb, Batch bt) {
-  constexpr int CH = fp_chunk<NB>(), PS = fp_ps<NB>();
+  constexpr int CH = fp_chunk<NB>();
   constexpr int R = fpg_rounds<G>(), NP = fpg_planes<G>(), FPG_THREADS = fpg_threads<G>();
   static_assert(G == FP_G, "slot tables are built for FP_G LORs per lane");
-  constexpr int ZN = NP * PS * NB, LN = 2 * LI_MAX;
+  static_assert(NB == 1 && CH == FPG_CH, "pair layout of one image");
+  constexpr int ZN = fpg_addr(NP), LN = 2 * LI_MAX;  // planes 0 .. NP - 1
   __shared__ __align__(16) float lds[ZN > LN ? ZN : LN];
   __shared__ float2 sr[CH];
   __shared__ double inv[R][FPG_THREADS];  // exact boundary spacing (0: direct)
@@ -588,11 +609,10 @@ __global__ __launch_bounds__(fpg_threads<G>()) void k_fprj_g(GeomView g, const f
   __shared__ SnCsr csr;
   load_sn_csr(csr, g, tid, FPG_THREADS);
   // guard planes (never written by the staging loop)
-  for (int i = tid; i < FP_GUARD * PS * NB; i += FPG_THREADS) zs[i] = 0.f;
-  for (int i = tid; i < fpg_top<G>() * PS * NB; i += FPG_THREADS)
-    zs[(FP_GUARD + ZSTRIDE) * PS * NB + i] = 0.f;
+  for (int i = tid; i < fpg_addr(FP_GUARD); i += FPG_THREADS) zs[i] = 0.f;
+  for (int i = fpg_addr(FP_GUARD + ZSTRIDE) + tid; i < ZN; i += FPG_THREADS) zs[i] = 0.f;
 
-  FpLor L[R];
+  FpLorG L[R];
   FpSumG<NB, G> A[R];
   bool two[R], live[R];
   int dr[R];  // d * 64 + r0 of the slot
@@ -614,8 +634,10 @@ __global__ __launch_bounds__(fpg_threads<G>()) void k_fprj_g(GeomView g, const f
     }
     z0[r][tid] = z.x;
     inv[r][tid] = iv;
-    L[r].ao = (ia + FP_GUARD) * PS * NB;
-    L[r].bo = (wsum - ia - 2 + FP_GUARD) * PS * NB;
+    const int pa = ia + FP_GUARD, pb = wsum - ia - 2 + FP_GUARD;
+    L[r].ao = fpg_addr(pa);
+    L[r].bo = fpg_addr(pb);
+    L[r].o = CH + (pa & 1);
     L[r].invf = (float)iv;
     static_for<G>([&](auto kc) {
       constexpr int k = decltype(kc)::value;
@@ -657,14 +679,15 @@ __global__ __launch_bounds__(fpg_threads<G>()) void k_fprj_g(GeomView g, const f
     for (int i = 0; i < NST; i++) {
       const int idx = tid + i * FPG_THREADS;
       if (idx < NE) {
-        const int j = idx / (ZSTRIDE * NB), rem = idx - j * (ZSTRIDE * NB);
-        const int z = rem / NB, b = rem - z * NB;
-        zs[((z + FP_GUARD) * PS + j) * NB + b] = pre[i];
+        const int j = idx / ZSTRIDE, z = idx - j * ZSTRIDE;
+        zs[fpg_addr(z + FP_GUARD) + j] = pre[i];
       }
     }
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      const int ia = L[r].ao / (PS * NB) - FP_GUARD;
+      // plane of LOR a: pa = 2 (ao / PP) + (odd plane of its pair)
+      const int pq = L[r].ao / FPG_PP;
+      const int ia = 2 * pq + (L[r].ao - pq * FPG_PP >= CH ? 1 : 0) - FP_GUARD;
       const double iv = inv[r][tid];
       L[r].tbr = iv > 0.0 ? (float)(((double)(ia + 1) - (double)z0[r][tid]) * iv - sb) : 3e38f;
     }
@@ -674,9 +697,9 @@ __global__ __launch_bounds__(fpg_threads<G>()) void k_fprj_g(GeomView g, const f
     for (int r = 0; r < R; r++) {
       if (!live[r]) continue;
       if (two[r])
-        fpg_walk<NB, G, true>(L[r], A[r], sr, zs, nch);
+        fpg_walk<G, true>(L[r], A[r], sr, zs, nch);
       else
-        fpg_walk<NB, G, false>(L[r], A[r], sr, zs, nch);
+        fpg_walk<G, false>(L[r], A[r], sr, zs, nch);
     }
   }
 
