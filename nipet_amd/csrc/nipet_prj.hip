@@ -648,32 +648,55 @@ __global__ __launch_bounds__(fpg_threads<G>()) __attribute__((amdgpu_waves_per_e
     two[r] = __any(iv > 0.0 && iv < (double)lmax);
   }
 
-  // z-columns of a chunk: each thread loads its NST values of the next chunk
-  // into registers before walking the current one, so the loads are in flight
-  // during the walk instead of between the two barriers
-  constexpr int NE = CH * ZSTRIDE * NB, NST = (NE + FPG_THREADS - 1) / FPG_THREADS;
+  // Software pipeline over chunks of CH segments. While chunk c is walked,
+  // each thread has the z-columns of chunk c + CH in flight (NST values in
+  // registers), and lanes tid < CH the segment records of chunk c + 2 CH.
+  // Records pass through LDS (rec, one buffer per chunk parity) one chunk
+  // before their columns are fetched, so neither the column addresses nor a
+  // chunk's positions wait on a dependent global load.
+  constexpr int NE = CH * ZSTRIDE, NST = (NE + FPG_THREADS - 1) / FPG_THREADS;
+  __shared__ SegRec rec[2][CH];
   float pre[NST];
-  auto fetch = [&](int c) {
+  auto fetch = [&](int c, const SegRec *rc) {  // rc: records of chunk c
 #pragma unroll
     for (int i = 0; i < NST; i++) {
       const int idx = tid + i * FPG_THREADS;
-      const int j = idx / (ZSTRIDE * NB), rem = idx - j * (ZSTRIDE * NB);
-      const int z = rem / NB, b = rem - z * NB, k = c + j;
+      const int j = idx / ZSTRIDE, z = idx - j * ZSTRIDE;
       float v = 0.f;
-      if (idx < NE && k < nseg && z < NZ) v = img[((int64_t)g.seg[beg + k].colz + z) * nbt + b0 + b];
+      if (idx < NE && c + j < nseg && z < NZ) v = img[((int64_t)rc[j].colz + z) * nbt + b0];
       pre[i] = v;
     }
   };
-  if (nseg > 0) fetch(0);
-  for (int c = 0; c < nseg; c += CH) {
+  // lanes tid < CH: record c + CH + tid of the chunk after the current one,
+  // held as scalars (a SegRec variable would be placed in LDS, and its store
+  // would wait for every load in flight)
+  double ns = 0.0;
+  float nl = 0.f;
+  int nz = 0;
+  auto load_rec = [&](int k) {
+    const SegRec *q = g.seg + beg + k;
+    ns = q->s;
+    nl = q->len;
+    nz = q->colz;
+  };
+  if (tid < CH && tid < nseg) rec[0][tid] = g.seg[beg + tid];
+  if (tid < CH && CH + tid < nseg) load_rec(CH + tid);
+  __syncthreads();
+  if (nseg > 0) fetch(0, rec[0]);
+  for (int c = 0, pb = 0; c < nseg; c += CH, pb ^= 1) {
     const int nch = nseg - c < CH ? nseg - c : CH;
+    __syncthreads();
     // positions are relative to the chunk's middle segment: float positions
     // within +-8 segments carry half the rounding of ones within 16
-    const double sb = g.seg[beg + c + (nch >> 1)].s;
-    __syncthreads();
+    const double sb = rec[pb][nch >> 1].s;
     if (tid < CH) {
-      const SegRec sg = g.seg[beg + (c + tid < nseg ? c + tid : c)];
-      sr[tid] = c + tid < nseg ? make_float2((float)(sg.s - sb), sg.len) : make_float2(-1e30f, 0.f);
+      sr[tid] = c + tid < nseg ? make_float2((float)(rec[pb][tid].s - sb), rec[pb][tid].len)
+                               : make_float2(-1e30f, 0.f);
+      if (c + CH + tid < nseg) {
+        rec[pb ^ 1][tid].s = ns;
+        rec[pb ^ 1][tid].len = nl;
+        rec[pb ^ 1][tid].colz = nz;
+      }
     }
 #pragma unroll
     for (int i = 0; i < NST; i++) {
@@ -692,7 +715,10 @@ __global__ __launch_bounds__(fpg_threads<G>()) __attribute__((amdgpu_waves_per_e
       L[r].tbr = iv > 0.0 ? (float)(((double)(ia + 1) - (double)z0[r][tid]) * iv - sb) : 3e38f;
     }
     __syncthreads();
-    if (c + CH < nseg) fetch(c + CH);
+    if (c + CH < nseg) {
+      fetch(c + CH, rec[pb ^ 1]);
+      if (tid < CH && c + 2 * CH + tid < nseg) load_rec(c + 2 * CH + tid);
+    }
 #pragma unroll
     for (int r = 0; r < R; r++) {
       if (!live[r]) continue;
