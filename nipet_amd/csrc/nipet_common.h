@@ -11,6 +11,9 @@ namespace nipet {
 constexpr int NX = 320;   // device image x (u)
 constexpr int NY = 320;   // device image y (v)
 constexpr int NZ = 127;   // device image z (w), fastest
+// segments of one transaxial ray: one per voxel column it crosses, at most
+// one more than the grid lines it crosses (checked when the geometry is built)
+constexpr int MAX_SEG = NX + NY + 1;
 constexpr int64_t NVOX = (int64_t)NX * NY * NZ;
 constexpr int MAXNSN = 4096;  // largest sinogram count handled (span-1: 4084)
 constexpr int LI_MAX = 2112;  // largest Michelogram (ring pair) count

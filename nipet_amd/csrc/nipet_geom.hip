@@ -458,6 +458,10 @@ static int geom_build(const NipetGeomDesc *d, NipetGeom *g) {
   std::vector<int32_t> off(d->naw + 1);
   int64_t tot = 0;
   for (int i = 0; i < d->naw; i++) {
+    if (cnt[i] > MAX_SEG) {  // the projectors stage one ray's list in LDS
+      set_error("transaxial ray %d has %d segments (at most %d)", i, cnt[i], MAX_SEG);
+      return NIPET_EINVAL;
+    }
     off[i] = (int32_t)tot;
     tot += cnt[i];
   }

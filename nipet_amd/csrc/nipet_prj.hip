@@ -181,6 +181,14 @@ __device__ inline void fma_nb(float (&acc)[NB], float p, const float (&x)[NB]) {
   for (int b = 0; b < NB; b++) acc[b] = fmaf(p, x[b], acc[b]);
 }
 
+// One bin's segment records [n] copied to LDS with 16-byte loads.
+__device__ inline void load_segs(SegRec *dst, const SegRec *__restrict__ src, int n, int tid,
+                                 int nthr) {
+  const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+  uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+  for (int i = tid; i < n; i += nthr) d4[i] = s4[i];
+}
+
 struct FpLor {
   float tbr;   // next z boundary of LOR a relative to the chunk's origin (float)
   float invf;  // boundary spacing (float)
@@ -354,22 +362,45 @@ __global__ __launch_bounds__(FP_THREADS, NB >= 4 ? 4 : 6) void k_fprj(GeomView g
     two[r] = __any(iv > 0.0 && iv < (double)lmax);
   }
 
+  // Software pipeline over chunks of CH segments: while chunk c is walked,
+  // each thread has its NST z-column values of chunk c + CH in flight. The
+  // bin's segment records sit in LDS (copied once), so neither the column
+  // addresses nor a chunk's positions wait on a global load.
+  constexpr int NE = CH * ZSTRIDE * NB, NST = (NE + FP_THREADS - 1) / FP_THREADS;
+  __shared__ SegRec segl[MAX_SEG];
+  load_segs(segl, g.seg + beg, nseg, tid, FP_THREADS);
+  float pre[NST];
+  auto fetch = [&](int c) {
+#pragma unroll
+    for (int i = 0; i < NST; i++) {
+      const int idx = tid + i * FP_THREADS;
+      const int j = idx / (ZSTRIDE * NB), rem = idx - j * (ZSTRIDE * NB);
+      const int z = rem / NB, b = rem - z * NB;
+      float v = 0.f;
+      if (idx < NE && c + j < nseg && z < NZ)
+        v = img[((int64_t)segl[c + j].colz + z) * nbt + b0 + b];
+      pre[i] = v;
+    }
+  };
+  __syncthreads();
+  if (nseg > 0) fetch(0);
   for (int c = 0; c < nseg; c += CH) {
     const int nch = nseg - c < CH ? nseg - c : CH;
+    __syncthreads();
     // positions are relative to the chunk's middle segment: float positions
     // within +-8 segments carry half the rounding of ones within 16
-    const double sb = g.seg[beg + c + (nch >> 1)].s;
-    __syncthreads();
-    if (tid < CH) {
-      const SegRec sg = g.seg[beg + (c + tid < nseg ? c + tid : c)];
-      sr[tid] = c + tid < nseg ? make_float2((float)(sg.s - sb), sg.len) : make_float2(-1e30f, 0.f);
-    }
-    for (int idx = tid; idx < CH * ZSTRIDE * NB; idx += FP_THREADS) {
-      const int j = idx / (ZSTRIDE * NB), rem = idx - j * (ZSTRIDE * NB);
-      const int z = rem / NB, b = rem - z * NB, k = c + j;
-      float v = 0.f;
-      if (k < nseg && z < NZ) v = img[((int64_t)g.seg[beg + k].colz + z) * nbt + b0 + b];
-      zs[((z + FP_GUARD) * PS + j) * NB + b] = v;
+    const double sb = segl[c + (nch >> 1)].s;
+    if (tid < CH)
+      sr[tid] = c + tid < nseg ? make_float2((float)(segl[c + tid].s - sb), segl[c + tid].len)
+                               : make_float2(-1e30f, 0.f);
+#pragma unroll
+    for (int i = 0; i < NST; i++) {
+      const int idx = tid + i * FP_THREADS;
+      if (idx < NE) {
+        const int j = idx / (ZSTRIDE * NB), rem = idx - j * (ZSTRIDE * NB);
+        const int z = rem / NB, b = rem - z * NB;
+        zs[((z + FP_GUARD) * PS + j) * NB + b] = pre[i];
+      }
     }
     // exact next-boundary positions relative to the chunk's middle segment
 #pragma unroll
@@ -379,6 +410,7 @@ __global__ __launch_bounds__(FP_THREADS, NB >= 4 ? 4 : 6) void k_fprj(GeomView g
       L[r].tbr = iv > 0.0 ? (float)(((double)(ia + 1) - (double)z0[r][tid]) * iv - sb) : 3e38f;
     }
     __syncthreads();
+    if (c + CH < nseg) fetch(c + CH);
 #pragma unroll
     for (int r = 0; r < FP_ROUNDS; r++) {
       if (!live[r]) continue;
@@ -648,56 +680,35 @@ __global__ __launch_bounds__(fpg_threads<G>()) __attribute__((amdgpu_waves_per_e
     two[r] = __any(iv > 0.0 && iv < (double)lmax);
   }
 
-  // Software pipeline over chunks of CH segments. While chunk c is walked,
-  // each thread has the z-columns of chunk c + CH in flight (NST values in
-  // registers), and lanes tid < CH the segment records of chunk c + 2 CH.
-  // Records pass through LDS (rec, one buffer per chunk parity) one chunk
-  // before their columns are fetched, so neither the column addresses nor a
-  // chunk's positions wait on a dependent global load.
+  // Software pipeline over chunks of CH segments: while chunk c is walked,
+  // each thread has its NST z-column values of chunk c + CH in flight. The
+  // bin's segment records sit in LDS (copied once), so neither the column
+  // addresses nor a chunk's positions wait on a global load.
   constexpr int NE = CH * ZSTRIDE, NST = (NE + FPG_THREADS - 1) / FPG_THREADS;
-  __shared__ SegRec rec[2][CH];
+  __shared__ SegRec segl[MAX_SEG];
+  load_segs(segl, g.seg + beg, nseg, tid, FPG_THREADS);
   float pre[NST];
-  auto fetch = [&](int c, const SegRec *rc) {  // rc: records of chunk c
+  auto fetch = [&](int c) {
 #pragma unroll
     for (int i = 0; i < NST; i++) {
       const int idx = tid + i * FPG_THREADS;
       const int j = idx / ZSTRIDE, z = idx - j * ZSTRIDE;
       float v = 0.f;
-      if (idx < NE && c + j < nseg && z < NZ) v = img[((int64_t)rc[j].colz + z) * nbt + b0];
+      if (idx < NE && c + j < nseg && z < NZ) v = img[((int64_t)segl[c + j].colz + z) * nbt + b0];
       pre[i] = v;
     }
   };
-  // lanes tid < CH: record c + CH + tid of the chunk after the current one,
-  // held as scalars (a SegRec variable would be placed in LDS, and its store
-  // would wait for every load in flight)
-  double ns = 0.0;
-  float nl = 0.f;
-  int nz = 0;
-  auto load_rec = [&](int k) {
-    const SegRec *q = g.seg + beg + k;
-    ns = q->s;
-    nl = q->len;
-    nz = q->colz;
-  };
-  if (tid < CH && tid < nseg) rec[0][tid] = g.seg[beg + tid];
-  if (tid < CH && CH + tid < nseg) load_rec(CH + tid);
   __syncthreads();
-  if (nseg > 0) fetch(0, rec[0]);
-  for (int c = 0, pb = 0; c < nseg; c += CH, pb ^= 1) {
+  if (nseg > 0) fetch(0);
+  for (int c = 0; c < nseg; c += CH) {
     const int nch = nseg - c < CH ? nseg - c : CH;
     __syncthreads();
     // positions are relative to the chunk's middle segment: float positions
     // within +-8 segments carry half the rounding of ones within 16
-    const double sb = rec[pb][nch >> 1].s;
-    if (tid < CH) {
-      sr[tid] = c + tid < nseg ? make_float2((float)(rec[pb][tid].s - sb), rec[pb][tid].len)
+    const double sb = segl[c + (nch >> 1)].s;
+    if (tid < CH)
+      sr[tid] = c + tid < nseg ? make_float2((float)(segl[c + tid].s - sb), segl[c + tid].len)
                                : make_float2(-1e30f, 0.f);
-      if (c + CH + tid < nseg) {
-        rec[pb ^ 1][tid].s = ns;
-        rec[pb ^ 1][tid].len = nl;
-        rec[pb ^ 1][tid].colz = nz;
-      }
-    }
 #pragma unroll
     for (int i = 0; i < NST; i++) {
       const int idx = tid + i * FPG_THREADS;
@@ -715,10 +726,7 @@ __global__ __launch_bounds__(fpg_threads<G>()) __attribute__((amdgpu_waves_per_e
       L[r].tbr = iv > 0.0 ? (float)(((double)(ia + 1) - (double)z0[r][tid]) * iv - sb) : 3e38f;
     }
     __syncthreads();
-    if (c + CH < nseg) {
-      fetch(c + CH, rec[pb ^ 1]);
-      if (tid < CH && c + 2 * CH + tid < nseg) load_rec(c + 2 * CH + tid);
-    }
+    if (c + CH < nseg) fetch(c + CH);
 #pragma unroll
     for (int r = 0; r < R; r++) {
       if (!live[r]) continue;
