@@ -117,6 +117,19 @@ __device__ inline void ldv(float (&r)[NB], const float *p) {
   }
 }
 
+template <int NB>
+__device__ inline void stv(float *p, const float (&r)[NB]) {
+  if constexpr (NB == 1) {
+    p[0] = r[0];
+  } else if constexpr (NB == 2) {
+    *reinterpret_cast<float2 *>(p) = make_float2(r[0], r[1]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < NB; k += 4)
+      *reinterpret_cast<float4 *>(p + k) = make_float4(r[k], r[k + 1], r[k + 2], r[k + 3]);
+  }
+}
+
 // Compile-time loop: f(std::integral_constant<int, i>) for i = 0 .. N-1, so
 // that register arrays are only ever indexed by constants.
 template <class F, int... I>
@@ -368,20 +381,30 @@ __global__ __launch_bounds__(FP_THREADS, NB >= 4 ? 3 : 6) void k_fprj(GeomView g
   // each thread has its NST z-column values of chunk c + CH in flight. The
   // bin's segment records sit in LDS (copied once), so neither the column
   // addresses nor a chunk's positions wait on a global load.
-  constexpr int NE = CH * ZSTRIDE * NB, NST = (NE + FP_THREADS - 1) / FP_THREADS;
+  // A thread stages whole (segment, z) groups of NB values: one vector load
+  // per group where the rows' replicates are aligned for it (every width of
+  // an aligned batch), NB scalar loads otherwise.
+  constexpr int NG = CH * ZSTRIDE, NSG = (NG + FP_THREADS - 1) / FP_THREADS;
   __shared__ SegRec segl[MAX_SEG];
   load_segs(segl, g.seg + beg, nseg, tid, FP_THREADS);
-  float pre[NST];
+  const bool vec = nbt % NB == 0 && b0 % NB == 0;  // grid-uniform
+  float pre[NSG][NB];
   auto fetch = [&](int c) {
 #pragma unroll
-    for (int i = 0; i < NST; i++) {
-      const int idx = tid + i * FP_THREADS;
-      const int j = idx / (ZSTRIDE * NB), rem = idx - j * (ZSTRIDE * NB);
-      const int z = rem / NB, b = rem - z * NB;
-      float v = 0.f;
-      if (idx < NE && c + j < nseg && z < NZ)
-        v = img[((int64_t)segl[c + j].colz + z) * nbt + b0 + b];
-      pre[i] = v;
+    for (int i = 0; i < NSG; i++) {
+      const int gi = tid + i * FP_THREADS;
+      const int j = gi / ZSTRIDE, z = gi - j * ZSTRIDE;
+#pragma unroll
+      for (int b = 0; b < NB; b++) pre[i][b] = 0.f;
+      if (gi < NG && c + j < nseg && z < NZ) {
+        const float *q = img + ((int64_t)segl[c + j].colz + z) * nbt + b0;
+        if (vec) {
+          ldv<NB>(pre[i], q);
+        } else {
+#pragma unroll
+          for (int b = 0; b < NB; b++) pre[i][b] = q[b];
+        }
+      }
     }
   };
   __syncthreads();
@@ -396,12 +419,11 @@ __global__ __launch_bounds__(FP_THREADS, NB >= 4 ? 3 : 6) void k_fprj(GeomView g
       sr[tid] = c + tid < nseg ? make_float2((float)(segl[c + tid].s - sb), segl[c + tid].len)
                                : make_float2(-1e30f, 0.f);
 #pragma unroll
-    for (int i = 0; i < NST; i++) {
-      const int idx = tid + i * FP_THREADS;
-      if (idx < NE) {
-        const int j = idx / (ZSTRIDE * NB), rem = idx - j * (ZSTRIDE * NB);
-        const int z = rem / NB, b = rem - z * NB;
-        zs[((z + FP_GUARD) * PS + j) * NB + b] = pre[i];
+    for (int i = 0; i < NSG; i++) {
+      const int gi = tid + i * FP_THREADS;
+      if (gi < NG) {
+        const int j = gi / ZSTRIDE, z = gi - j * ZSTRIDE;
+        stv<NB>(zs + ((z + FP_GUARD) * PS + j) * NB, pre[i]);
       }
     }
     // exact next-boundary positions relative to the chunk's middle segment
