@@ -158,7 +158,8 @@ int nipet_osem_corr(float *rho, float *an, const float *a, const float *n, const
  * subset; mask: uint8 image; win: NULL (every segment of every ray) or the
  * mask's segment windows from nipet_osem_window, valid when img is zero
  * outside the mask (after any earlier update with that mask) - the projectors
- * then skip the segments whose columns miss the mask; work_sino: float
+ * then skip the segments whose columns miss the mask (a window reaching
+ * outside its bin's list is clamped into it); work_sino: float
  * [nprj][nsn]; work_img: float image distinct from img and sens (aliased work
  * buffers are rejected). */
 int nipet_osem_subset(const NipetGeom *g, float *img, const uint16_t *prompts, const float *rho,

@@ -86,14 +86,18 @@ struct Batch {
   const int32_t *win = nullptr;
 };
 
-// The bin's segment list, or its window.
+// The bin's segment list, or its window. A window is clamped into the list
+// (nipet_osem_window's always lies inside it), so that the kernels' LDS copy
+// of a bin's records (at most MAX_SEG) holds whatever a caller passes.
 __device__ inline void seg_range(const GeomView &g, const Batch &bt, int aw, int &beg,
                                  int &nseg) {
   beg = g.off[aw];
-  nseg = g.off[aw + 1] - beg;
+  const int n = g.off[aw + 1] - beg;
+  nseg = n;
   if (bt.win) {
-    beg += bt.win[2 * aw];
-    nseg = bt.win[2 * aw + 1];
+    const int f = min(max(bt.win[2 * aw], 0), n);
+    beg += f;
+    nseg = min(max(bt.win[2 * aw + 1], 0), n - f);
   }
 }
 
