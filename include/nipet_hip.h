@@ -67,6 +67,15 @@ void nipet_geom_destroy(NipetGeom *g);
 /* Sizes of the built geometry (host outputs, any may be NULL). */
 int nipet_geom_info(const NipetGeom *g, int *naw, int64_t *nseg_total, int *nli, int *nsn);
 
+/* The bins subs[0..n) (device int32) written to out[0..n) (device, distinct
+ * from subs) in order of decreasing segment count, bins outside 0 .. naw-1
+ * last; bins of equal count in no particular order. A projector dispatches
+ * workgroups in bin order, so the long rays start first. No counterpart in
+ * the reference (its OSEM walks subsets in table order); a device pass, so
+ * that preparing a reconstruction from device subsets needs no host copy. */
+int nipet_bins_longest_first(const NipetGeom *g, const int32_t *subs, int n, int32_t *out,
+                             void *stream);
+
 /* Copy the transaxial intersection lists to host arrays: offsets[naw+1],
  * col[nseg_total] (voxel column u + 320 v), t[nseg_total] (segment start, cm),
  * len[nseg_total] (segment length, cm). The equivalent of the reference's

@@ -90,6 +90,7 @@ def lib():
             'nipet_tune_get_run': ([I, P, P], I),
             'nipet_osem_subset': ([P, P, P, P, P, P, P, P, I, P, P, P], I),
             'nipet_osem_window': ([P, P, I, P, P], I),
+            'nipet_bins_longest_first': ([P, P, I, P, P], I),
             'nipet_osem_subset_sens': ([P, P, P, P, P, P, P, P, P, I, P, P, P, P], I),
             'nipet_osem_iter': ([P, P, P, P, P, P, P, P, I, I, P, P, P, P], I),
             'nipet_fprj_b': ([P, P, P, P, I, I, I, P], I),

@@ -234,6 +234,41 @@ __global__ void k_tx_lists(const float *crs, const short2 *s2c, int naw, double 
   }
 }
 
+// Bins of a subset ordered by decreasing segment count (bins outside
+// 0 .. naw-1 last), one workgroup: a counting sort by count with LDS buckets.
+// Bins of equal count keep no particular order: the order only decides which
+// bins are dispatched first (long rays first: a shorter tail).
+__global__ __launch_bounds__(1024) void k_bins_longest_first(const int32_t *__restrict__ off,
+                                                             int naw,
+                                                             const int32_t *__restrict__ subs,
+                                                             int n, int32_t *__restrict__ out) {
+  constexpr int NK = MAX_SEG + 2;  // bucket MAX_SEG - count; MAX_SEG + 1: invalid bins
+  __shared__ int cnt[NK];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < NK; i += blockDim.x) cnt[i] = 0;
+  __syncthreads();
+  auto bucket = [&](int aw) {
+    if (aw < 0 || aw >= naw) return MAX_SEG + 1;
+    const int c = off[aw + 1] - off[aw];
+    return MAX_SEG - min(max(c, 0), MAX_SEG);
+  };
+  for (int i = tid; i < n; i += blockDim.x) atomicAdd(&cnt[bucket(subs[i])], 1);
+  __syncthreads();
+  if (tid == 0) {  // exclusive scan (643 entries)
+    int acc = 0;
+    for (int k = 0; k < NK; k++) {
+      const int c = cnt[k];
+      cnt[k] = acc;
+      acc += c;
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += blockDim.x) {
+    const int aw = subs[i];
+    out[atomicAdd(&cnt[bucket(aw)], 1)] = aw;
+  }
+}
+
 }  // namespace nipet
 
 using namespace nipet;
@@ -534,6 +569,18 @@ int nipet_geom_info(const NipetGeom *g, int *naw, int64_t *nseg_total, int *nli,
   if (nseg_total) *nseg_total = g->nseg;
   if (nli) *nli = g->nli;
   if (nsn) *nsn = g->nsn;
+  return NIPET_OK;
+}
+
+int nipet_bins_longest_first(const NipetGeom *g, const int32_t *subs, int n, int32_t *out,
+                             void *stream) {
+  if (!g || n < 0 || (n > 0 && (!subs || !out)) || subs == out) {
+    set_error("nipet_bins_longest_first: bad argument (geometry, n >= 0, distinct subs/out)");
+    return NIPET_EINVAL;
+  }
+  if (n == 0) return NIPET_OK;
+  k_bins_longest_first<<<1, 1024, 0, (hipStream_t)stream>>>(g->off, g->naw, subs, n, out);
+  NIPET_HIP(hipGetLastError());
   return NIPET_OK;
 }
 

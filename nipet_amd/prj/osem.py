@@ -39,8 +39,17 @@ def _longest_first(geom, subs):
     """A subset's bins ordered by decreasing segment count (device int32).
     Workgroups are dispatched in bin order, so the long rays start first and
     the last wave of workgroups holds short ones (a shorter tail). FP and BP
-    walk the same list, so the OSEM result does not depend on the order."""
+    walk the same list, so the OSEM result does not depend on the order.
+    Device subsets are ordered on the device (no host copy, which would wait
+    for all queued work); host ones on the host."""
     torch = _torch()
+    if isinstance(subs, torch.Tensor) and subs.is_cuda:
+        src = subs.to(torch.int32).contiguous()
+        out = torch.empty_like(src)
+        _lib.check(_lib.lib().nipet_bins_longest_first(geom.handle, _lib.ptr(src), src.numel(),
+                                                       _lib.ptr(out), _lib.stream_handle()),
+                   'nipet_bins_longest_first')
+        return out
     rows = subs.cpu().numpy() if isinstance(subs, torch.Tensor) else np.asarray(subs)
     segc = geom.seg_counts()
     valid = (rows >= 0) & (rows < len(segc))

@@ -49,6 +49,8 @@ def test_status_codes_without_gpu(L):
     assert b'nipet_osem_subset_sens: NULL geometry' in L.nipet_last_error()
     assert L.nipet_osem_window(None, None, 0, None, None) == -1
     assert b'nipet_osem_window: NULL geometry' in L.nipet_last_error()
+    assert L.nipet_bins_longest_first(None, None, 5, None, None) == -1
+    assert b'nipet_bins_longest_first: bad argument' in L.nipet_last_error()
     assert L.nipet_osem_iter(None, None, None, None, None, None, None, None, 1, 2, None, None,
                              None, None) == -1
     assert b'nipet_osem_iter: NULL geometry' in L.nipet_last_error()

@@ -156,6 +156,25 @@ def test_osem_window_clamped_to_list(torch, params, subsets):
         assert rel < 1e-6, rel
 
 
+def test_bins_longest_first_device(torch, params, subsets):
+    """nipet_bins_longest_first (device subsets are ordered without a host
+    copy): the same bins as the host ordering, segment counts non-increasing,
+    bins outside the sinogram last."""
+    from nipet_amd.geom import get_geom
+    from nipet_amd.prj.osem import _longest_first
+    g = get_geom(params)
+    segc = g.seg_counts()
+    rows = np.concatenate([subsets[3], [-1, g.naw + 5], subsets[3][:7]]).astype(np.int32)
+    got = _longest_first(g, torch.from_numpy(rows).cuda()).cpu().numpy()
+    host = _longest_first(g, rows).cpu().numpy()
+    assert np.array_equal(np.sort(got), np.sort(rows))
+    valid = (got >= 0) & (got < g.naw)
+    assert not valid[-2:].any() and valid[:-2].all()
+    key = segc[got[:-2]]
+    assert (np.diff(key) <= 0).all()
+    assert np.array_equal(key, segc[host[:-2]])   # same counts in the same order
+
+
 def test_osem_span1_mini_vs_oracle(torch, params, subsets):
     """OSEM on span-1 data (4084 sinograms, one LOR per bin; the sensitivity
     images are made before the loop in this span) against the FP64 oracle
