@@ -564,8 +564,10 @@ struct FpLorG {
   int o;
 };
 
+// srx / srl: position and length of the chunk's segment j in lane j (every
+// wave holds the chunk's records), broadcast to the scalar unit per segment
 template <int G, bool TWO>
-__device__ inline void fpg_walk(FpLorG &l, FpSumG<1, G> &acc, const float2 *__restrict__ sr,
+__device__ inline void fpg_walk(FpLorG &l, FpSumG<1, G> &acc, float srx, float srl,
                                 const float *__restrict__ zs, int nch) {
   constexpr int PP = FPG_PP;
   for (int j8 = 0; j8 < nch; j8 += SUBSUM) {
@@ -574,7 +576,7 @@ __device__ inline void fpg_walk(FpLorG &l, FpSumG<1, G> &acc, const float2 *__re
     int o = l.o;
 #pragma unroll 4
     for (int jj = 0; jj < SUBSUM; jj++) {
-      const float2 sl = sr[j8 + jj];
+      const float2 sl = make_float2(rdlane(srx, j8 + jj), rdlane(srl, j8 + jj));
       if constexpr (!TWO) {
         const float q = l.tbr - sl.x;
         const float p1 = __builtin_amdgcn_fmed3f(q, 0.f, sl.y);
@@ -647,10 +649,9 @@ __global__ __launch_bounds__(fpg_threads<G>()) __attribute__((amdgpu_waves_per_e
   static_assert(NB == 1 && CH == FPG_CH, "pair layout of one image");
   constexpr int ZN = fpg_addr(NP), LN = 2 * LI_MAX;  // planes 0 .. NP - 1
   __shared__ __align__(16) float lds[ZN > LN ? ZN : LN];
-  __shared__ float2 sr[CH];
   __shared__ double inv[R][FPG_THREADS];  // exact boundary spacing (0: direct)
   __shared__ float z0[R][FPG_THREADS];    // zeta of the slot's first LOR at t = 0
-  const int tid = threadIdx.x, wave = tid >> 6;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int row = blockIdx.x;
   const int aw = subs ? subs[row] : row;
   const int nsn = g.nsn, nbt = bt.nbt, b0 = bt.b0;
@@ -734,9 +735,10 @@ __global__ __launch_bounds__(fpg_threads<G>()) __attribute__((amdgpu_waves_per_e
     // positions are relative to the chunk's middle segment: float positions
     // within +-8 segments carry half the rounding of ones within 16
     const double sb = segl[c + (nch >> 1)].s;
-    if (tid < CH)
-      sr[tid] = c + tid < nseg ? make_float2((float)(segl[c + tid].s - sb), segl[c + tid].len)
-                               : make_float2(-1e30f, 0.f);
+    // lanes 0 .. CH-1 of every wave: the chunk's segment positions / lengths
+    const int jl = lane < CH ? lane : 0;
+    const float srx = c + jl < nseg ? (float)(segl[c + jl].s - sb) : -1e30f;
+    const float srl = c + jl < nseg ? segl[c + jl].len : 0.f;
 #pragma unroll
     for (int i = 0; i < NST; i++) {
       const int idx = tid + i * FPG_THREADS;
@@ -759,9 +761,9 @@ __global__ __launch_bounds__(fpg_threads<G>()) __attribute__((amdgpu_waves_per_e
     for (int r = 0; r < R; r++) {
       if (!live[r]) continue;
       if (two[r])
-        fpg_walk<G, true>(L[r], A[r], sr, zs, nch);
+        fpg_walk<G, true>(L[r], A[r], srx, srl, zs, nch);
       else
-        fpg_walk<G, false>(L[r], A[r], sr, zs, nch);
+        fpg_walk<G, false>(L[r], A[r], srx, srl, zs, nch);
     }
   }
 
