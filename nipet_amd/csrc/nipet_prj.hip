@@ -945,6 +945,8 @@ __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g
   const double inv_lane = lane > 0 ? atn / (2.0 * lane) : 0.0;
   const float sf_lane = lor_sfac(2.f * lane, g.vox_z, atn);
   const int q = lane;
+  const char *tbyte = (const char *)tab;
+  const int qb = 2 * q;  // lane's byte offset in a table row
   // direct planes: plane 2q <- L * va0[q]
   float X0[NB];
   ldv<NB>(X0, (const float *)(srb + ta[bp_row(0) + 1 + q]));
@@ -1016,12 +1018,26 @@ __global__ __launch_bounds__(BP_THREADS, NB == 1 ? 6 : 4) void k_bprj(GeomView g
         // 0 <= w0 <= 2d (the run word's start and steps keep it there) puts
         // q - h and q - e in -d .. 63, inside row d
         const int h = w0 >> 1, e = (2 * d - w0) >> 1;
-        const int ka = rb - h + q, kb = rb - e + q;
         float X[NB], Y[NB], Xb[NB], Yb[NB];
-        ldv<NB>(X, (const float *)(srb + ta[ka]));
-        ldv<NB>(Y, (const float *)(srb + ta[ka - 1]));
-        ldv<NB>(Xb, (const float *)(srb + tb[kb]));
-        ldv<NB>(Yb, (const float *)(srb + tb[kb + 1]));
+        if constexpr (NB >= 4) {
+          // ta[ka - 1], ta[ka] and tb[kb], tb[kb + 1]: one per-lane byte
+          // offset (2 q) plus a wave-uniform one per pair, the pair's second
+          // entry at an immediate offset (2 VALU instead of 6; the scalar
+          // side grows, which one and two images cannot spare: measured
+          // 1.352 -> 1.387 ms for one image, 0.822 -> 0.792 ms per image at 4)
+          const uint16_t *pa = (const uint16_t *)(tbyte + qb + 2 * (rb - h - 1));
+          const uint16_t *pb = (const uint16_t *)(tbyte + qb + 2 * (BP_TABN + rb - e));
+          ldv<NB>(Y, (const float *)(srb + pa[0]));
+          ldv<NB>(X, (const float *)(srb + pa[1]));
+          ldv<NB>(Xb, (const float *)(srb + pb[0]));
+          ldv<NB>(Yb, (const float *)(srb + pb[1]));
+        } else {
+          const int ka = rb - h + q, kb = rb - e + q;
+          ldv<NB>(X, (const float *)(srb + ta[ka]));
+          ldv<NB>(Y, (const float *)(srb + ta[ka - 1]));
+          ldv<NB>(Xb, (const float *)(srb + tb[kb]));
+          ldv<NB>(Yb, (const float *)(srb + tb[kb + 1]));
+        }
 #pragma unroll
         for (int b = 0; b < NB; b++) {
           S1[b] = X[b] + Xb[b];
