@@ -391,7 +391,8 @@ __global__ __launch_bounds__(FP_THREADS, NB >= 4 ? 3 : 6) void k_fprj(GeomView g
   constexpr int NG = CH * ZSTRIDE, NSG = (NG + FP_THREADS - 1) / FP_THREADS;
   __shared__ SegRec segl[MAX_SEG];
   load_segs(segl, g.seg + beg, nseg, tid, FP_THREADS);
-  const bool vec = nbt % NB == 0 && b0 % NB == 0;  // grid-uniform
+  const bool vec = nbt % NB == 0 && b0 % NB == 0 &&
+                   (uintptr_t)img % (sizeof(float) * NB) == 0;  // grid-uniform
   float pre[NSG][NB];
   auto fetch = [&](int c) {
 #pragma unroll
